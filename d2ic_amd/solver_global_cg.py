@@ -1,0 +1,185 @@
+"""Global NLCG solver for mesh DIC on the GPU.
+
+Drop-in for ``GlobalCGSolver`` (reference ``d2ic/solver_global_cg.py:35-198``).
+It minimises J(u) = 1/2 mean_p (g(x_p + u(x_p)) - f(x_p))^2 + reg_strength *
+E_spring(u) with nonlinear CG (Polak-Ribiere+, restart on non-descent,
+10-trial Armijo backtracking with c = 1e-4, stop when |grad J|_2 < tol before
+the update) -- the reference's ``_cg_solve`` (``:326-469``) -- in one
+persistent HIP kernel per solve (``d2ic_cg_solve``).
+
+``interpolation``: "cubic" (Keys a = -1/2 cubic convolution with clamp-to-edge
+boundaries; the reference delegates cubic to im_jax, whose kernel is not
+available) or "linear"/"bilinear" (the reference's in-repo bilinear sampler,
+``_bilinear_sample``, ``:481-509``).
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+
+from . import _lib
+from .mesh_assets import MeshAssets, PixelAssets
+from .solver_base import SolverBase
+from .types import Array
+
+
+@dataclass(frozen=True)
+class GlobalCGResult:
+    """``solver_global_cg.py:27-32``: u (Nn,2), placeholder strain (Nn,2),
+    history (max_iter,3) [J, |grad|, alpha] or None, iteration count."""
+    u_nodal: Array
+    strain: Array
+    history: Array | None = None
+    n_iters: int | None = None
+
+
+def _normalize_interpolation(interpolation: str) -> str:
+    """``solver_global_cg.py:472-478``."""
+    value = str(interpolation).lower()
+    if value == "bilinear":
+        value = "linear"
+    if value not in ("cubic", "linear"):
+        raise ValueError(f"Unsupported interpolation: {interpolation}")
+    return value
+
+
+class CGPlan:
+    """Owner of a device plan (``d2ic_cg_plan``) built from PixelAssets."""
+
+    def __init__(self, pix: PixelAssets):
+        t = _lib.require_gpu()
+        self._keep = []
+
+        def dev(a, dtype):
+            x = _lib.to_dev(a, dtype)
+            self._keep.append(x)
+            return x
+
+        coords = dev(pix.pixel_coords_ref, t.float32)
+        pnodes = dev(pix.pixel_nodes, t.int32)
+        shapeN = dev(pix.pixel_shapeN, t.float32)
+        pelts = dev(pix.pixel_elements, t.int32) if getattr(pix, "pixel_elements", None) is not None else None
+        nb_idx = dev(pix.node_neighbor_index, t.int32)
+        nb_deg = dev(pix.node_neighbor_degree, t.int32)
+        nb_w = dev(pix.node_neighbor_weight, t.float32)
+        reg_w = dev(pix.node_reg_weight, t.float32)
+        self.n_pixels = int(coords.shape[0])
+        self.n_nodes = int(nb_deg.shape[0])
+        md = int(nb_idx.shape[1]) if nb_idx.dim() == 2 else 0
+        h = ctypes.c_void_p(0)
+        _lib.call("d2ic_cg_plan_create", _lib.ptr(coords), _lib.ptr(pnodes), _lib.ptr(shapeN), _lib.ptr(pelts),
+                  self.n_pixels, self.n_nodes, _lib.ptr(nb_idx), _lib.ptr(nb_deg), _lib.ptr(nb_w), _lib.ptr(reg_w),
+                  md, ctypes.byref(h), _lib.stream())
+        self.handle = h
+        self._ref_key = None
+        self._keep = []  # the plan copied what it needs
+
+    def info(self):
+        a, b, c, d = (ctypes.c_int64(0) for _ in range(4))
+        _lib.call("d2ic_cg_plan_info", self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                  ctypes.byref(d))
+        return dict(n_pixels=a.value, n_nodes=b.value, n_tiles=c.value, device_bytes=d.value)
+
+    def set_reference(self, ref_image, interpolation: str):
+        t = _lib.torch()
+        ref = _lib.to_dev(ref_image, t.float32)
+        key = (ref.data_ptr(), tuple(ref.shape), interpolation, getattr(ref, "_version", 0))
+        if key == self._ref_key:
+            return ref
+        H, W = int(ref.shape[0]), int(ref.shape[1])
+        _lib.call("d2ic_cg_set_reference", self.handle, _lib.ptr(ref), H, W, _lib.INTERP[interpolation],
+                  _lib.stream())
+        self._ref_key = key
+        self._ref = ref
+        return ref
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().d2ic_cg_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = ctypes.c_void_p(0)
+
+
+class GlobalCGSolver(SolverBase):
+    """See module docstring.  Same constructor as the reference
+    (``solver_global_cg.py:45-63``)."""
+
+    def __init__(self, verbose: bool = False, interpolation: str = "cubic", use_map_coordinates: bool | None = None,
+                 log_cg: bool = False) -> None:
+        self._compiled = False
+        self._plan: CGPlan | None = None
+        self._verbose = bool(verbose)
+        self._log_cg = bool(log_cg)
+        if use_map_coordinates is not None:
+            warnings.warn(
+                "use_map_coordinates is kept for backward compatibility; use interpolation='cubic' or 'linear'.",
+                DeprecationWarning,
+                stacklevel=2,
+            )
+            interpolation = "cubic" if use_map_coordinates else "linear"
+        self._interpolation = _normalize_interpolation(interpolation)
+        self.last_pass_count: int | None = None
+
+    @property
+    def interpolation(self) -> str:
+        return self._interpolation
+
+    def compile(self, assets: Any) -> None:
+        pixel_data = getattr(assets, "pixel_data", None)
+        if pixel_data is None:
+            raise ValueError("MeshAssets must provide pixel_data for GlobalCGSolver.")
+        self._plan = CGPlan(pixel_data)
+        self._compiled = True
+
+    def warmup(self, state: Any) -> None:
+        """Samples f at the pixels once; there is nothing to JIT-compile."""
+        if not self._compiled or self._plan is None:
+            raise RuntimeError("GlobalCGSolver.compile() must be called before warmup().")
+        if state.assets.pixel_data is None:
+            raise ValueError("MeshAssets must provide pixel_data for GlobalCGSolver warmup.")
+        self._plan.set_reference(state.ref_image, self._interpolation)
+
+    @property
+    def plan(self) -> CGPlan | None:
+        return self._plan
+
+    def solve(self, state: Any, def_image: Array) -> GlobalCGResult:
+        if not self._compiled or self._plan is None:
+            raise RuntimeError("GlobalCGSolver.compile() must be called before solve().")
+        t = _lib.torch()
+        plan = self._plan
+        plan.set_reference(state.ref_image, self._interpolation)
+        g = _lib.to_dev(def_image, t.float32)
+        if g.dim() != 2:
+            raise ValueError("def_image must be a 2-D array (H, W)")
+        nn = plan.n_nodes
+        if state.u0_nodal is not None:
+            u = _lib.to_dev(state.u0_nodal, t.float32, (nn, 2)).clone()
+        else:
+            u = t.zeros((nn, 2), dtype=t.float32, device=_lib.device())
+        cfg = state.config
+        max_iter = int(cfg.max_iters)
+        hist = t.empty((max(max_iter, 0), 3), dtype=t.float32, device=_lib.device())
+        counters = t.zeros((2,), dtype=t.int32, device=_lib.device())
+        _lib.call("d2ic_cg_solve", plan.handle, _lib.ptr(g), int(g.shape[0]), int(g.shape[1]), _lib.ptr(u),
+                  float(cfg.reg_strength), max_iter, float(cfg.tol), _lib.ptr(hist), _lib.ptr(counters),
+                  _lib.ptr(counters) + 4, _lib.stream())
+        n_iters, passes = (int(v) for v in counters.cpu().tolist())
+        if n_iters < 0:
+            raise RuntimeError("d2ic_cg_solve did not finish (a grid barrier timed out)")
+        self.last_pass_count = passes
+        if self._verbose and self._log_cg:
+            h = hist[:max(n_iters + 1, 1)].cpu().numpy()
+            for k, (J, gn, a) in enumerate(h):
+                if np.isfinite(J):
+                    print(f"CG iter {k}: J={J:.3e}, |grad|={gn:.3e}, alpha={a:.3e}")
+        save_history = bool(getattr(cfg, "save_history", False))
+        return GlobalCGResult(u_nodal=u, strain=t.zeros_like(u), history=hist if save_history else None,
+                              n_iters=n_iters)

@@ -1,0 +1,153 @@
+/*
+ * d2ic_amd.h -- C ABI of the MI355X-native global FE-DIC hot path.
+ *
+ * Every entry point takes plain device pointers (allocated by the caller;
+ * the Python layer uses torch tensors for ownership), sizes, and a HIP stream
+ * passed as void* (NULL = default stream).  Calls are asynchronous on that
+ * stream unless the comment says they return a host value (those synchronise
+ * the stream).  Every call returns a status code; d2ic_last_error() gives the
+ * thread-local message of the last failure.
+ *
+ * The reference (D2IC 1.0.1) is pure Python/JAX and has no FFI; each entry
+ * point names the reference function whose behaviour it replaces.
+ */
+#ifndef D2IC_AMD_H
+#define D2IC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D2IC_OK 0
+#define D2IC_ERR_INVALID 1 /* invalid argument (Python: ValueError) */
+#define D2IC_ERR_NOMEM 2   /* device allocation / budget exhausted (Python: RuntimeError) */
+#define D2IC_ERR_DEVICE 3  /* HIP runtime or kernel failure (Python: RuntimeError) */
+#define D2IC_ERR_STATE 4   /* lifecycle error, e.g. no reference set (Python: RuntimeError) */
+#define D2IC_ERR_TIMEOUT 5 /* a persistent kernel's bounded wait expired (Python: RuntimeError) */
+
+#define D2IC_INTERP_CUBIC 0
+#define D2IC_INTERP_LINEAR 1
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* d2ic_last_error(void);
+/* ABI version (major*100 + minor). */
+int d2ic_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Pixel assets (replaces build_pixel_assets, d2ic/pixel_assets.py:21-112)
+ * ------------------------------------------------------------------------ */
+
+/* Candidate stage: for every pixel of the bbox rectangle rows [i0,i1) x cols
+ * [j0,j1) (row-major, _sample_pixels_in_bbox, pixel_assets.py:115-132) apply
+ * the ROI filter (roi: HxW uint8 or NULL, pixel_assets.py:69-73) and the
+ * first-match element search (build_pixel_to_element_mapping_numpy,
+ * pixel_assets.py:135-237).  cand_elt: (i1-i0)*(j1-j0) int32, -1 when the pixel
+ * is dropped.  Writes the number of kept pixels to *n_kept (host; synchronises). */
+int d2ic_pixel_map(const float* nodes, int64_t n_nodes, const int32_t* elements, int64_t n_elems, int32_t H,
+                   int32_t W, int32_t i0, int32_t i1, int32_t j0, int32_t j1, const uint8_t* roi, int32_t* cand_elt,
+                   int64_t* n_kept, void* stream);
+
+/* Fill stage: stable compaction of the kept candidates (row-major order), then
+ * pixel_coords_ref (np,2) f32, pixel_nodes (np,4) i32, pixel_elements (np,) i32,
+ * roi_mask_flat (np,) i32 (pixel_assets.py:76-98) and pixel_shapeN (np,4) f32
+ * from 10 Newton steps (compute_pixel_shape_functions_jax, pixel_assets.py:240-294). */
+int d2ic_pixel_fill(const int32_t* cand_elt, int32_t H, int32_t W, int32_t i0, int32_t i1, int32_t j0, int32_t j1,
+                    const float* nodes, const int32_t* elements, int64_t n_pixels, float* pixel_coords,
+                    int32_t* pixel_nodes, float* pixel_shapeN, int32_t* pixel_elements, int32_t* roi_flat,
+                    void* stream);
+
+/* Node->pixel table, degrees: node_pixel_degree (n_nodes,) and the maximum
+ * degree (host; synchronises) (build_node_pixel_dense, pixel_assets.py:459-498). */
+int d2ic_node_pixel_degree(const int32_t* pixel_nodes, int64_t n_pixels, int64_t n_nodes, int32_t* degree,
+                           int32_t* max_degree, void* stream);
+
+/* Node->pixel table, fill: node_pixel_index (n_nodes,max_degree) i32 padded
+ * with -1 and node_N_weight f32 padded with 0; each node lists its pixels in
+ * ascending pixel order, i.e. the slot order of the sequential scan in
+ * _build_node_pixel_dense_jax (pixel_assets.py:501-526). */
+int d2ic_node_pixel_table(const int32_t* pixel_nodes, const float* pixel_shapeN, int64_t n_pixels, int64_t n_nodes,
+                          int32_t max_degree, int32_t* node_pixel_index, float* node_N_weight, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Global NLCG solver (replaces GlobalCGSolver / _cg_solve,
+ * d2ic/solver_global_cg.py:35-469)
+ * ------------------------------------------------------------------------ */
+typedef struct d2ic_cg_plan d2ic_cg_plan;
+
+/* Build the solver's internal layout from PixelAssets arrays: pixels grouped
+ * by element into tiles of <= 64 pixels (structure-of-arrays), the
+ * node->tile gather table, and the spring-regularisation stencil
+ * (node_neighbor_* and node_reg_weight, solver_global_cg.py:261-286).
+ * pixel_elements may be NULL (elements are then identified by their node tuple). */
+int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, const float* pixel_shapeN,
+                        const int32_t* pixel_elements, int64_t n_pixels, int64_t n_nodes,
+                        const int32_t* nb_index, const int32_t* nb_degree, const float* nb_weight,
+                        const float* reg_weight, int32_t nb_max_degree, d2ic_cg_plan** plan, void* stream);
+int d2ic_cg_plan_destroy(d2ic_cg_plan* plan);
+
+/* Sample the reference image f at every pixel once (the reference resamples
+ * it in every pass, solver_global_cg.py:221,225).  interp: D2IC_INTERP_*. */
+int d2ic_cg_set_reference(d2ic_cg_plan* plan, const float* ref_image, int32_t H, int32_t W, int32_t interp,
+                          void* stream);
+
+/* Sizes of the plan's layout (for budgeting and roofline accounting). */
+int d2ic_cg_plan_info(const d2ic_cg_plan* plan, int64_t* n_pixels, int64_t* n_nodes, int64_t* n_tiles,
+                      int64_t* device_bytes);
+
+/* One solve (one problem per launch): u (n_nodes,2) f32 holds u0 on entry and
+ * the solution on exit.  hist (max_iter,3) f32 may be NULL; rows
+ * [J_trial, |grad|, alpha], NaN padded.  n_iters: one int32 on the device.
+ * pass_count (may be NULL): one int32 on the device, number of pixel passes run. */
+int d2ic_cg_solve(d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W, float* u, float alpha_reg,
+                  int32_t max_iter, float tol, float* hist, int32_t* n_iters, int32_t* pass_count, void* stream);
+
+/* Many independent solves in one persistent launch with per-problem
+ * convergence masks (no reference counterpart; the batched form of
+ * d2ic_cg_solve).  All arrays have n_problems entries; hist/pass_count
+ * entries may be NULL.  A plan must not appear twice in one batch. */
+int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const float* const* def_images,
+                        const int32_t* H, const int32_t* W, float* const* u, const float* alpha_reg,
+                        const int32_t* max_iter, const float* tol, float* const* hist, int32_t* const* n_iters,
+                        int32_t* const* pass_count, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Translation ZNCC (replaces _match_centers_jax, solver_translation_zncc.py:118-168)
+ * ------------------------------------------------------------------------ */
+/* centers (n,2) f32; ref (Hr,Wr) and def (Hd,Wd) f32 row-major.  u_out (n,2)
+ * integer-valued f32 shifts, score_out (n,) f32 (NaN when the template or the
+ * search patch leaves the image). */
+int d2ic_zncc_match(const float* centers, int64_t n_centers, const float* ref, int32_t Hr, int32_t Wr,
+                    const float* def, int32_t Hd, int32_t Wd, int32_t win, int32_t search, float* u_out,
+                    float* score_out, void* stream);
+
+/* Inverse-distance weighting of centre shifts onto nodes, power 2
+ * (_project_centers_to_nodes_idw, dic_init_motion.py:64-91). */
+int d2ic_idw(const float* nodes, int64_t n_nodes, const float* centers, const float* disp, int64_t n_centers,
+             float* u_out, void* stream);
+
+/* Nodal LSQ Green-Lagrange strain (compute_green_lagrange_strain_nodes_lsq +
+ * green_lagrange_to_voigt, strain.py:11-114).  F_out/E_out (n,2,2) may be NULL;
+ * voigt (n,3). */
+int d2ic_strain(const float* u, const float* nodes, int64_t n_nodes, const int32_t* nb_index,
+                const int32_t* nb_degree, int32_t max_degree, float gauge_length, float eps, float* F_out,
+                float* E_out, float* voigt, void* stream);
+
+/* Reference-domain residual map (compute_discrepancy_map_ref,
+ * discrepancy.py:86-127): bilinear with NaN outside, averaged per image
+ * pixel.  map_out (H*W) f32; rms written to *rms (host; synchronises). */
+int d2ic_discrepancy_map(const float* ref_image, const float* def_image, int32_t H, int32_t W, const float* u,
+                         const float* pixel_coords, const int32_t* pixel_nodes, const float* pixel_shapeN,
+                         const int32_t* roi_flat, int64_t n_pixels, float* map_out, float* rms, void* stream);
+
+/* Synthetic speckle (workload generator, no reference counterpart): sum of
+ * isotropic Gaussian spots at pixel centres, each truncated at 4 sigma.
+ * spots: (n,4) f64 [cx, cy, sigma, amp]; out (H*W) f64. */
+int d2ic_render_speckle(const double* spots, int64_t n_spots, int32_t H, int32_t W, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D2IC_AMD_H */

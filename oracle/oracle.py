@@ -1,0 +1,333 @@
+"""CPU oracle for the D2IC global FE-DIC hot path -- TEST INFRASTRUCTURE ONLY.
+
+This module wraps ``oracle/_build/liboracle.so`` (built from ``d2ic_oracle.c``
+by ``oracle/Makefile``).  It is the checker: only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it, and never as the thing measured or shipped.  The product package
+``d2ic_amd`` does not import it.
+
+Every function takes ``dtype`` = ``np.float32`` (the reference's default JAX
+FP32 run) or ``np.float64`` (its x64 run).
+
+What pins each function (tests named in brackets):
+
+* ``pixel_candidates`` / ``pixel_to_element`` -- the reference's exact integer
+  fixture (``tests/test_pixel_assets.py:6-36``) [test_oracle_assets], brute
+  force vs. binned search on random convex quads, and the closed form on
+  structured grids (element = cell index) [test_oracle_assets].
+* ``shape_functions`` -- closed form on axis-aligned rectangles
+  (xi = 2(x - xc)/w), partition of unity, non-negativity, and
+  x(xi, eta) = p on distorted quads [test_oracle_assets].
+* ``node_pixel_dense`` -- ascending-pixel closed form on structured grids,
+  degree sum = 4 Np, and a numpy argsort restatement [test_oracle_assets].
+* ``node_neighbor_dense`` -- numpy unique/lexsort restatement and the exact
+  weights 1/(16 + 1e-6), 1/(16 sqrt 2 + 1e-6) [test_oracle_assets].
+* ``sample`` (linear) -- the in-repo formula: exact at pixel centres, exact
+  on affine images, clamp at the border [test_oracle_sampling].
+  ``sample`` (cubic) -- PARITY UNPINNED against the reference: the reference
+  calls im_jax.flat_nd_cubic_interpolate, which is not available.  The Keys
+  a=-1/2 kernel is pinned only by its own invariants: interpolation at
+  centres, reproduction of constants/ramps/quadratics, clamp boundary, and
+  derivative vs. central differences [test_oracle_sampling].
+* ``objective`` -- closed forms (zero residual at the true translation of an
+  affine image; E_reg = 0 for uniform u; reg gradient = L_W u) and gradient
+  vs. central finite differences in FP64 [test_oracle_objective].
+* ``cg_solve`` -- recovers a known rigid translation on synthetic speckle,
+  n_iters/history semantics (stop before update, NaN padding, alpha = 1 on
+  the first accepted trial) and FP32 vs FP64 agreement [test_oracle_cg].
+* ``zncc`` -- the reference's known-shift fixture
+  (``tests/test_solver_translation_zncc.py:8-33``) and a numpy brute force
+  [test_oracle_zncc].
+* ``idw`` -- closed form on tiny inputs and a numpy restatement [test_oracle_strain_idw].
+* ``strain`` -- the reference's closed form (``tests/test_strain_smoke.py:10-40``)
+  and exact E for affine fields [test_oracle_strain_idw].
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "_build" / "liboracle.so"
+_lib = None
+
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_dbl = ctypes.c_double
+
+
+def build(force: bool = False) -> Path:
+    """Compile the oracle with its Makefile (gcc)."""
+    if force or not _LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(_LIB_PATH))
+        for sfx, real in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
+            getattr(_lib, f"orc_objective_{sfx}").restype = _dbl
+            getattr(_lib, f"orc_pixel_candidates_{sfx}").restype = _i64
+            getattr(_lib, f"orc_first_element_{sfx}").restype = _i32
+    return _lib
+
+
+def _sfx(dtype) -> str:
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return "f32"
+    if dtype == np.float64:
+        return "f64"
+    raise ValueError(f"unsupported dtype {dtype}")
+
+
+def _real(dtype):
+    return ctypes.c_float if np.dtype(dtype) == np.float32 else ctypes.c_double
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads_f32())
+
+
+# --------------------------------------------------------------------------
+# assets
+# --------------------------------------------------------------------------
+
+def pixel_to_element(pixel_xy, nodes, elements, dtype=np.float32, use_bins=True):
+    s = _sfx(dtype)
+    xy = _c(pixel_xy, dtype).reshape(-1, 2)
+    nd = _c(nodes, dtype).reshape(-1, 2)
+    el = _c(elements, np.int32).reshape(-1, 4)
+    n = xy.shape[0]
+    elts = np.empty(n, np.int32)
+    pn = np.empty((n, 4), np.int32)
+    rc = getattr(lib(), f"orc_pixel_to_element_{s}")(_p(xy), _i64(n), _p(nd), _p(el), _i64(el.shape[0]),
+                                                        _int(int(use_bins)), _p(elts), _p(pn))
+    if rc != 0:
+        raise RuntimeError("orc_pixel_to_element failed")
+    return elts, pn
+
+
+def shape_functions(pixel_xy, pixel_nodes, nodes, dtype=np.float32):
+    s = _sfx(dtype)
+    xy = _c(pixel_xy, dtype).reshape(-1, 2)
+    pn = _c(pixel_nodes, np.int32).reshape(-1, 4)
+    nd = _c(nodes, dtype).reshape(-1, 2)
+    n = xy.shape[0]
+    N = np.empty((n, 4), dtype)
+    xe = np.empty((n, 2), dtype)
+    getattr(lib(), f"orc_shape_functions_{s}")(_p(xy), _p(pn), _i64(n), _p(nd), _p(N), _p(xe))
+    return N, xe
+
+
+def pixel_candidates(nodes, elements, H, W, roi_mask=None, pad=2.0, dtype=np.float32, use_bins=True):
+    """Candidate sampling + ROI filter + element assignment + drop
+    (pixel_assets.py:58-79).  Returns coords, pixel_nodes, pixel_elts, roi_flat."""
+    s = _sfx(dtype)
+    nd = _c(nodes, dtype).reshape(-1, 2)
+    el = _c(elements, np.int32).reshape(-1, 4)
+    cap = int(H) * int(W)
+    coords = np.empty((cap, 2), dtype)
+    pn = np.empty((cap, 4), np.int32)
+    pe = np.empty(cap, np.int32)
+    rf = np.empty(cap, np.int64)
+    roi = None
+    if roi_mask is not None:
+        roi = np.ascontiguousarray(np.asarray(roi_mask, dtype=bool).astype(np.uint8))
+    n = getattr(lib(), f"orc_pixel_candidates_{s}")(_p(nd), _i64(nd.shape[0]), _p(el), _i64(el.shape[0]),
+                                                      _int(int(H)), _int(int(W)), _p(roi), _dbl(float(pad)),
+                                                      _int(int(use_bins)), _i64(cap), _p(coords), _p(pn), _p(pe),
+                                                      _p(rf))
+    if n < 0:
+        raise RuntimeError("orc_pixel_candidates failed")
+    return coords[:n].copy(), pn[:n].copy(), pe[:n].copy(), rf[:n].copy()
+
+
+def node_pixel_dense(pixel_nodes, shapeN, n_nodes, dtype=np.float32):
+    s = _sfx(dtype)
+    pn = _c(pixel_nodes, np.int32).reshape(-1, 4)
+    N = _c(shapeN, dtype).reshape(-1, 4)
+    Np = pn.shape[0]
+    deg = np.empty(n_nodes, np.int32)
+    md = _i64(0)
+    f = getattr(lib(), f"orc_node_pixel_dense_{s}")
+    f(_p(pn), _p(N), _i64(Np), _i64(n_nodes), _i64(0), _p(deg), ctypes.byref(md), None, None)
+    maxdeg = int(md.value)
+    idx = np.empty((n_nodes, maxdeg), np.int32)
+    w = np.empty((n_nodes, maxdeg), dtype)
+    if maxdeg > 0:
+        f(_p(pn), _p(N), _i64(Np), _i64(n_nodes), _i64(maxdeg), _p(deg), ctypes.byref(md), _p(idx), _p(w))
+    return idx, w, deg
+
+
+def node_neighbor_dense(elements, nodes, n_nodes, dtype=np.float32):
+    s = _sfx(dtype)
+    el = _c(elements, np.int32).reshape(-1, 4)
+    nd = _c(nodes, dtype).reshape(-1, 2)
+    cap = 64
+    while True:
+        idx = -np.ones((n_nodes, cap), np.int32)
+        w = np.zeros((n_nodes, cap), dtype)
+        deg = np.empty(n_nodes, np.int32)
+        md = _i64(0)
+        rc = getattr(lib(), f"orc_node_neighbor_dense_{s}")(_p(el), _i64(el.shape[0]), _p(nd), _i64(n_nodes),
+                                                              _i64(cap), _p(idx), _p(deg), _p(w), ctypes.byref(md))
+        if rc == 0:
+            m = int(md.value)
+            return idx[:, :m].copy(), deg, w[:, :m].copy()
+        cap *= 2
+
+
+def pixel_assets(nodes, elements, image_shape, roi_mask=None, pad=2.0, dtype=np.float32):
+    """Full build_pixel_assets restatement (pixel_assets.py:21-112) as a dict."""
+    H, W = image_shape
+    nodes = np.asarray(nodes, dtype)
+    coords, pn, pe, rf = pixel_candidates(nodes, elements, H, W, roi_mask=roi_mask, pad=pad, dtype=dtype)
+    N, _ = shape_functions(coords, pn, nodes, dtype=dtype)
+    nn = nodes.shape[0]
+    nb_idx, nb_deg, nb_w = node_neighbor_dense(elements, nodes, nn, dtype=dtype)
+    npi, npw, npd = node_pixel_dense(pn, N, nn, dtype=dtype)
+    return dict(
+        pixel_coords_ref=coords, pixel_nodes=pn, pixel_shapeN=N, pixel_elts=pe,
+        node_neighbor_index=nb_idx, node_neighbor_degree=nb_deg, node_neighbor_weight=nb_w,
+        node_reg_weight=np.ones(nn, dtype),
+        node_pixel_index=npi, node_N_weight=npw, node_pixel_degree=npd,
+        roi_mask_flat=rf, image_shape=(int(H), int(W)),
+    )
+
+
+# --------------------------------------------------------------------------
+# sampling / objective / NLCG
+# --------------------------------------------------------------------------
+
+_INTERP = {"cubic": 0, "linear": 1, "bilinear": 1}
+
+
+def sample(image, xy, interp="linear", dtype=np.float32, with_grad=False):
+    s = _sfx(dtype)
+    img = _c(image, dtype)
+    H, W = img.shape
+    pts = _c(xy, dtype).reshape(-1, 2)
+    n = pts.shape[0]
+    v = np.empty(n, dtype)
+    gx = np.empty(n, dtype) if with_grad else None
+    gy = np.empty(n, dtype) if with_grad else None
+    getattr(lib(), f"orc_sample_{s}")(_p(img), _i64(H), _i64(W), _p(pts), _i64(n), _int(_INTERP[interp]),
+                                      _p(v), _p(gx), _p(gy))
+    return (v, gx, gy) if with_grad else v
+
+
+def _asset_args(a, dtype):
+    coords = _c(a["pixel_coords_ref"], dtype)
+    pn = _c(a["pixel_nodes"], np.int32)
+    N = _c(a["pixel_shapeN"], dtype)
+    npi = _c(a["node_pixel_index"], np.int32)
+    npw = _c(a["node_N_weight"], dtype)
+    npd = _c(a["node_pixel_degree"], np.int32)
+    nbi = _c(a["node_neighbor_index"], np.int32)
+    nbd = _c(a["node_neighbor_degree"], np.int32)
+    nbw = _c(a["node_neighbor_weight"], dtype)
+    rw = _c(a["node_reg_weight"], dtype)
+    keep = (coords, pn, N, npi, npw, npd, nbi, nbd, nbw, rw)
+    args = [_p(coords), _p(pn), _p(N), _i64(coords.shape[0]), _p(npi), _p(npw), _p(npd),
+            _i64(npi.shape[1] if npi.ndim == 2 else 0), _p(nbi), _p(nbd), _p(nbw), _p(rw),
+            _i64(nbi.shape[1] if nbi.ndim == 2 else 0), _i64(rw.shape[0])]
+    return keep, args
+
+
+def objective(u, ref_image, def_image, assets, alpha_reg=0.0, interp="cubic", dtype=np.float32, with_grad=True):
+    s = _sfx(dtype)
+    R = _real(dtype)
+    uu = _c(u, dtype).reshape(-1, 2)
+    f = _c(ref_image, dtype)
+    g = _c(def_image, dtype)
+    H, W = f.shape
+    keep, args = _asset_args(assets, dtype)
+    grad = np.empty_like(uu) if with_grad else None
+    J = getattr(lib(), f"orc_objective_{s}")(_p(uu), _p(f), _p(g), _i64(H), _i64(W), _int(_INTERP[interp]),
+                                             R(alpha_reg), *args, _p(grad))
+    return (float(J), grad) if with_grad else float(J)
+
+
+def cg_solve(u0, ref_image, def_image, assets, alpha_reg=0.0, max_iter=200, tol=1e-3, interp="cubic",
+             dtype=np.float32):
+    """_cg_solve restatement.  Returns (u, hist (max_iter,3), n_iters)."""
+    s = _sfx(dtype)
+    R = _real(dtype)
+    u = np.array(u0, dtype=dtype).reshape(-1, 2).copy()
+    f = _c(ref_image, dtype)
+    g = _c(def_image, dtype)
+    H, W = f.shape
+    keep, args = _asset_args(assets, dtype)
+    hist = np.empty((int(max_iter), 3), dtype)
+    nit = ctypes.c_int(0)
+    getattr(lib(), f"orc_cg_solve_{s}")(_p(u), _p(f), _p(g), _i64(H), _i64(W), _int(_INTERP[interp]), R(alpha_reg),
+                                        _int(int(max_iter)), R(tol), *args, _p(hist), ctypes.byref(nit))
+    return u, hist, int(nit.value)
+
+
+# --------------------------------------------------------------------------
+# ZNCC / IDW / strain
+# --------------------------------------------------------------------------
+
+def zncc(centers, ref_image, def_image, win, search, dtype=np.float32):
+    s = _sfx(dtype)
+    c = _c(centers, dtype).reshape(-1, 2)
+    r = _c(ref_image, dtype)
+    d = _c(def_image, dtype)
+    u = np.empty((c.shape[0], 2), dtype)
+    sc = np.empty(c.shape[0], dtype)
+    getattr(lib(), f"orc_zncc_{s}")(_p(c), _i64(c.shape[0]), _p(r), _i64(r.shape[0]), _i64(r.shape[1]), _p(d),
+                                    _i64(d.shape[0]), _i64(d.shape[1]), _int(int(win)), _int(int(search)), _p(u),
+                                    _p(sc))
+    return u, sc
+
+
+def idw(nodes, centers, disp, dtype=np.float32):
+    s = _sfx(dtype)
+    nd = _c(nodes, dtype).reshape(-1, 2)
+    c = _c(centers, dtype).reshape(-1, 2)
+    d = _c(disp, dtype).reshape(-1, 2)
+    valid = ~np.isnan(d).any(axis=1)
+    c = np.ascontiguousarray(c[valid])
+    d = np.ascontiguousarray(d[valid])
+    out = np.zeros_like(nd)
+    if c.shape[0] == 0:
+        return out
+    getattr(lib(), f"orc_idw_{s}")(_p(nd), _i64(nd.shape[0]), _p(c), _p(d), _i64(c.shape[0]), _p(out))
+    return out
+
+
+def strain(u, nodes, nb_idx, nb_deg, gauge_length=0.0, eps=1e-8, dtype=np.float32):
+    """Returns (F (Nn,2,2), E (Nn,2,2), voigt (Nn,3))."""
+    s = _sfx(dtype)
+    R = _real(dtype)
+    uu = _c(u, dtype).reshape(-1, 2)
+    X = _c(nodes, dtype).reshape(-1, 2)
+    bi = _c(nb_idx, np.int32)
+    bd = _c(nb_deg, np.int32)
+    n = X.shape[0]
+    F = np.empty((n, 2, 2), dtype)
+    E = np.empty((n, 2, 2), dtype)
+    V = np.empty((n, 3), dtype)
+    getattr(lib(), f"orc_strain_{s}")(_p(uu), _p(X), _i64(n), _p(bi), _p(bd), _i64(bi.shape[1]), R(gauge_length),
+                                      R(eps), _p(F), _p(E), _p(V))
+    return F, E, V

@@ -1,0 +1,93 @@
+"""ZNCC, IDW, strain and discrepancy kernels vs the reference's fixtures and the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_zncc_reference_fixture(gpu):
+    """tests/test_solver_translation_zncc.py:8-33 of the reference."""
+    from d2ic_amd.solver_translation_zncc import match_centers
+
+    print("[case] zncc fixture", flush=True)
+    rng = np.random.default_rng(0)
+    ref = rng.normal(size=(64, 64)).astype(np.float32)
+    dx, dy = 3, 2
+    de = np.zeros_like(ref)
+    de[dy:, dx:] = ref[:-dy, :-dx]
+    centers = np.array([[32.0, 32.0], [40.0, 24.0], [24.0, 40.0]], np.float32)
+    u, s = match_centers(centers, ref, de, win=9, search=6)
+    u = u.cpu().numpy()
+    s = s.cpu().numpy()
+    assert np.isfinite(s).all()
+    assert np.allclose(u, [dx, dy], atol=0.5)
+    uo, so = O.zncc(centers, ref, de, 9, 6)
+    assert np.array_equal(u, uo)
+    np.testing.assert_allclose(s, so, rtol=1e-5)
+
+
+def test_zncc_out_of_bounds_and_speckle(gpu):
+    from d2ic_amd import speckle
+    from d2ic_amd.solver_translation_zncc import match_centers
+
+    print("[case] zncc speckle", flush=True)
+    ref, de = speckle.speckle_pair(128, 128, 5, speckle.rigid(3.2, -1.7), noise_sigma=1.0)
+    rng = np.random.default_rng(1)
+    centers = rng.uniform(0, 128, size=(64, 2)).astype(np.float32)
+    u, s = match_centers(centers, ref, de, win=21, search=6)
+    uo, so = O.zncc(centers, ref, de, 21, 6)
+    u, s = u.cpu().numpy(), s.cpu().numpy()
+    assert np.array_equal(np.isnan(s), np.isnan(so))
+    assert np.array_equal(u, uo)
+    ok = ~np.isnan(so)
+    np.testing.assert_allclose(s[ok], so[ok], rtol=1e-5, atol=1e-6)
+    assert np.all(u[~ok] == 0)
+
+
+def test_idw_matches_oracle(gpu):
+    from d2ic_amd.dic_init_motion import project_centers_to_nodes_idw
+
+    rng = np.random.default_rng(2)
+    nodes = rng.uniform(0, 100, size=(300, 2)).astype(np.float32)
+    centers = rng.uniform(0, 100, size=(77, 2)).astype(np.float32)
+    centers[3] = nodes[10]  # exact hit is copied
+    disp = rng.normal(size=(77, 2)).astype(np.float32)
+    out = project_centers_to_nodes_idw(nodes, centers, disp).cpu().numpy()
+    ref = O.idw(nodes, centers, disp)
+    np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(out[10], disp[3])
+
+
+def test_strain_reference_closed_form(gpu):
+    """tests/test_strain_smoke.py:10-40 of the reference."""
+    from d2ic_amd import Mesh, build_node_neighbor_tables, compute_green_lagrange_strain_nodes_lsq, \
+        green_lagrange_to_voigt
+
+    nodes = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]], np.float32)
+    mesh = Mesh(nodes_xy=nodes, elements=np.array([[0, 1, 3, 2]], np.int32))
+    idx, deg = build_node_neighbor_tables(mesh)
+    eps = 0.01
+    disp = np.stack([nodes[:, 0] * eps, np.zeros(4, np.float32)], axis=1)
+    _, E = compute_green_lagrange_strain_nodes_lsq(disp, nodes, idx, deg)
+    v = green_lagrange_to_voigt(E).cpu().numpy()
+    np.testing.assert_allclose(v[:, 0], eps + 0.5 * eps ** 2, atol=5e-4)
+    np.testing.assert_allclose(v[:, 1:], 0.0, atol=5e-4)
+
+
+def test_strain_matches_oracle(gpu):
+    from d2ic_amd import build_node_neighbor_tables, mask_to_mesh, small_strain_nodes_lsq
+
+    mesh = mask_to_mesh(np.ones((100, 120), bool), 16)
+    rng = np.random.default_rng(3)
+    u = rng.normal(scale=0.3, size=mesh.nodes_xy.shape).astype(np.float32)
+    idx, deg = build_node_neighbor_tables(mesh)
+    for L in (0.0, 20.0):
+        v = small_strain_nodes_lsq(u, mesh.nodes_xy, idx, deg, gauge_length=L).cpu().numpy()
+        _, _, vo = O.strain(u, mesh.nodes_xy, idx, deg, gauge_length=L)
+        _, _, v64 = O.strain(u, mesh.nodes_xy, idx, deg, gauge_length=L, dtype=np.float64)
+        np.testing.assert_allclose(v, vo, atol=1e-6)
+        np.testing.assert_allclose(v, v64, atol=1e-6)
