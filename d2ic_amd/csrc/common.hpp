@@ -65,4 +65,59 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Pointers read from descriptor structs are generic; accessing them through
+// the global address space gives global_load/store (vmcnt only) instead of
+// flat accesses that also count against lgkmcnt and so serialise with LDS.
+#define D2IC_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const D2IC_GLOBAL T* gld(const T* p) {
+    return (const D2IC_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ D2IC_GLOBAL T* gst(T* p) {
+    return (D2IC_GLOBAL T*)p;
+}
+
+// Wave64 sums through DPP (VALU, no LDS): quad_perm [1,0,3,2] and [2,3,0,1],
+// row_half_mirror, row_mirror reduce every 16-lane row; row_bcast15 and
+// row_bcast31 (rows masked) fold the rows.  The total is exact in lane 63
+// only; the association order is fixed.
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, Ctrl, RowMask, 0xF, false);
+}
+
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(dpp_i<Ctrl, RowMask>(__float_as_int(v)));
+}
+
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<Ctrl, RowMask>((int)(b & 0xffffffffll));
+    const int hi = dpp_i<Ctrl, RowMask>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ float wave_sum63(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    v += dpp_f<0x142, 0xA>(v);
+    v += dpp_f<0x143, 0xC>(v);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum63(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    v += dpp_d<0x142, 0xA>(v);
+    v += dpp_d<0x143, 0xC>(v);
+    return v;
+}
+
 }  // namespace d2ic

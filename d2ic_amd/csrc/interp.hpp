@@ -11,20 +11,23 @@ namespace d2ic {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Samplers take the sample position as an integer cell index plus a
+// fractional offset in [0, 1): position - 0.5 = x0 + fx.  Callers that know the
+// reference coordinate of a pixel split it as integer part + small offset so
+// the fractional part keeps full FP32 precision even where |x| ~ 4096 (an
+// absolute FP32 coordinate there is quantised to ~2.4e-4 px).
+
 // Bilinear sampler of _bilinear_sample (solver_global_cg.py:481-509) with its
 // clip bounds: the x index is clipped by H-1 and the y index by W-1 (the bounds
 // come from the transposed image), then the gather clamps x by W-1 and y by
 // H-1.  Returns value and coordinate derivative (floor contributes nothing).
-__device__ __forceinline__ void sample_linear(const float* __restrict__ img, int H, int W, float x, float y,
-                                              float& v, float& gx, float& gy) {
-    const float xs = x - 0.5f, ys = y - 0.5f;
-    const float x0f = floorf(xs), y0f = floorf(ys);
-    const int x0 = (int)x0f, y0 = (int)y0f;
-    const float fx = xs - x0f, fy = ys - y0f;
+__device__ __forceinline__ void sample_linear_at(const float* __restrict__ img_, int H, int W, int x0, float fx,
+                                                 int y0, float fy, float& v, float& gx, float& gy) {
+    const __attribute__((address_space(1))) float* img = (const __attribute__((address_space(1))) float*)img_;
     const int xa = clampi(clampi(x0, 0, H - 1), 0, W - 1), xb = clampi(clampi(x0 + 1, 0, H - 1), 0, W - 1);
     const int ya = clampi(clampi(y0, 0, W - 1), 0, H - 1), yb = clampi(clampi(y0 + 1, 0, W - 1), 0, H - 1);
-    const float I00 = img[ya * W + xa], I10 = img[ya * W + xb];
-    const float I01 = img[yb * W + xa], I11 = img[yb * W + xb];
+    const float I00 = img[(size_t)ya * W + xa], I10 = img[(size_t)ya * W + xb];
+    const float I01 = img[(size_t)yb * W + xa], I11 = img[(size_t)yb * W + xb];
     const float ofx = 1.0f - fx, ofy = 1.0f - fy;
     v = ofx * ofy * I00 + fx * ofy * I10 + ofx * fy * I01 + fx * fy * I11;
     gx = ofy * (I10 - I00) + fy * (I11 - I01);
@@ -46,21 +49,19 @@ __device__ __forceinline__ void keys_weights(float t, float w[4], float dw[4]) {
     dw[3] = 0.5f * (3.0f * t2 - 2.0f * t);
 }
 
-__device__ __forceinline__ void sample_cubic(const float* __restrict__ img, int H, int W, float x, float y, float& v,
-                                             float& gx, float& gy) {
-    const float xs = x - 0.5f, ys = y - 0.5f;
-    const float x0f = floorf(xs), y0f = floorf(ys);
-    const int x0 = (int)x0f, y0 = (int)y0f;
+__device__ __forceinline__ void sample_cubic_at(const float* __restrict__ img_, int H, int W, int x0, float tx, int y0,
+                                                float ty, float& v, float& gx, float& gy) {
+    const __attribute__((address_space(1))) float* img = (const __attribute__((address_space(1))) float*)img_;
     float wx[4], dwx[4], wy[4], dwy[4];
-    keys_weights(xs - x0f, wx, dwx);
-    keys_weights(ys - y0f, wy, dwy);
+    keys_weights(tx, wx, dwx);
+    keys_weights(ty, wy, dwy);
     int cx[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) cx[j] = clampi(x0 - 1 + j, 0, W - 1);
     float acc = 0.f, accx = 0.f, accy = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float* row = img + (size_t)clampi(y0 - 1 + i, 0, H - 1) * W;
+        const __attribute__((address_space(1))) float* row = img + (size_t)clampi(y0 - 1 + i, 0, H - 1) * W;
         float rv = 0.f, rd = 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -77,12 +78,76 @@ __device__ __forceinline__ void sample_cubic(const float* __restrict__ img, int 
     gy = accy;
 }
 
+__device__ __forceinline__ void sample_at(int interp, const float* __restrict__ img, int H, int W, int x0, float fx,
+                                          int y0, float fy, float& v, float& gx, float& gy) {
+    if (interp == 0)
+        sample_cubic_at(img, H, W, x0, fx, y0, fy, v, gx, gy);
+    else
+        sample_linear_at(img, H, W, x0, fx, y0, fy, v, gx, gy);
+}
+
+// Tap index clamps of the two samplers (see sample_linear_at / sample_cubic_at).
+__device__ __forceinline__ int clamp_col(int interp, int x, int H, int W) {
+    return interp == 0 ? clampi(x, 0, W - 1) : clampi(clampi(x, 0, H - 1), 0, W - 1);
+}
+__device__ __forceinline__ int clamp_row(int interp, int y, int H, int W) {
+    return interp == 0 ? clampi(y, 0, H - 1) : clampi(clampi(y, 0, W - 1), 0, H - 1);
+}
+
+// Same samplers reading a staged box: box[(r) * bw + c] holds
+// img[clamp_row(by0 + r), clamp_col(bx0 + c)]; (lx, ly) = (x0 - bx0, y0 - by0).
+__device__ __forceinline__ void sample_box(int interp, const __attribute__((address_space(3))) float* box, int bw, int lx,
+                                           float fx, int ly, float fy, float& v, float& gx, float& gy) {
+    if (interp == 0) {
+        float wx[4], dwx[4], wy[4], dwy[4];
+        keys_weights(fx, wx, dwx);
+        keys_weights(fy, wy, dwy);
+        float acc = 0.f, accx = 0.f, accy = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const __attribute__((address_space(3))) float* row = box + (ly - 1 + i) * bw + (lx - 1);
+            float rv = 0.f, rd = 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float I = row[j];
+                rv += wx[j] * I;
+                rd += dwx[j] * I;
+            }
+            acc += wy[i] * rv;
+            accx += wy[i] * rd;
+            accy += dwy[i] * rv;
+        }
+        v = acc;
+        gx = accx;
+        gy = accy;
+    } else {
+        const __attribute__((address_space(3))) float* r0 = box + ly * bw + lx;
+        const float I00 = r0[0], I10 = r0[1], I01 = r0[bw], I11 = r0[bw + 1];
+        const float ofx = 1.0f - fx, ofy = 1.0f - fy;
+        v = ofx * ofy * I00 + fx * ofy * I10 + ofx * fy * I01 + fx * fy * I11;
+        gx = ofy * (I10 - I00) + fy * (I11 - I01);
+        gy = ofx * (I01 - I00) + fx * (I11 - I10);
+    }
+}
+
+// Split of a pixel coordinate plus displacement: (x - 0.5) + u = x0 + f with
+// x = xi + xf (xi integer), evaluated as xi + ((xf - 0.5) + u).
+__device__ __forceinline__ void split_coord(float x, float u, int& x0, float& f) {
+    const float xi = floorf(x);
+    const float t = (x - xi - 0.5f) + u;
+    const float ti = floorf(t);
+    x0 = (int)xi + (int)ti;
+    f = t - ti;
+}
+
+// Sample at an absolute coordinate (reference form: floor(x - 0.5)).
 __device__ __forceinline__ void sample(int interp, const float* __restrict__ img, int H, int W, float x, float y,
                                        float& v, float& gx, float& gy) {
-    if (interp == 0)
-        sample_cubic(img, H, W, x, y, v, gx, gy);
-    else
-        sample_linear(img, H, W, x, y, v, gx, gy);
+    int x0, y0;
+    float fx, fy;
+    split_coord(x, 0.f, x0, fx);
+    split_coord(y, 0.f, y0, fy);
+    sample_at(interp, img, H, W, x0, fx, y0, fy, v, gx, gy);
 }
 
 }  // namespace d2ic

@@ -11,7 +11,9 @@
 //     gradient is assembled deterministically (no float atomics);
 //   * the spring stencil with the combined weight w_ij * (c_i + c_j) / 2.
 //
-// Each iteration is two grid-wide steps separated by a grid barrier:
+// Each iteration is two grid-wide steps separated by a tree barrier whose last
+// arriving block (the root) reduces the step, decides every problem and
+// publishes the plan of the next step:
 //   A-step: every tile evaluates the residual at the candidate
 //           u + alpha * d (d formed on the fly from g, d_prev and beta) and
 //           writes its partial gradient (8 floats) and its sum of r^2; node
@@ -27,9 +29,10 @@
 //
 // Reductions are deterministic and independent of the batch composition:
 // per-tile sums of r^2 are converted exactly to a two-word fixed-point integer
-// (resolution 2^-40) and added with integer atomics; node-chunk partials are
-// reduced in fixed order by the last chunk to finish.  Dot products and the
-// objective are accumulated in FP64 over FP32 arithmetic.
+// (resolution 2^-40), pre-summed per block in LDS and added with integer
+// atomics spread over 16 slots; node-chunk partials are stored per chunk and
+// summed in fixed order by the root.  Dot products and the objective are
+// accumulated in FP64 over FP32 arithmetic.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -43,9 +46,12 @@
 namespace d2ic {
 
 constexpr int kTile = 64;
-constexpr int kBlock = 256;
+constexpr int kBlock = 1024;  // 16 waves: one block per CU
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxProblems = 256;
+constexpr int kSlots = 16;    // spread of the fixed-point residual sums
+constexpr int kGroup = 16;    // barrier fan-in
+constexpr int kMaxGroups = 64;
 constexpr double kFixScale = 1099511627776.0;  // 2^40
 
 enum Mode : int { MODE_INIT = 0, MODE_PASS = 1, MODE_NODE = 2, MODE_DONE = 3 };
@@ -59,9 +65,9 @@ enum Flag : int {
 };
 
 struct ProbDesc {
-    const int4* tile_hdr;    // {pixel start, count, group, 0}
-    const int4* tile_nodes;  // 4 node ids
-    const float* px;
+    const int32_t* tile_cnt;  // pixels in the tile (<= 64)
+    const int4* tile_nodes;   // 4 node ids
+    const float* px;          // padded: pixel (t, lane) at t * 64 + lane
     const float* py;
     const float4* N4;
     const float* F;
@@ -73,6 +79,7 @@ struct ProbDesc {
     float* G[2];
     float* D[2];
     float* TP;
+    float* TV;
     double* NCP;
     const float* img;
     float* hist;
@@ -84,67 +91,48 @@ struct ProbDesc {
     double invNp;
 };
 
+// Solver state of one problem; read and written only by the root block.
 struct ProbState {
     int mode, k, trial, restart, final_eval, cur, dcur, passes;
     float alpha, beta, gnorm, Jtrial;
     double J, gg;
 };
 
-struct Accum {
-    unsigned long long hi[kMaxProblems];
-    unsigned long long lo[kMaxProblems];
-    unsigned cnt[kMaxProblems];
-    int bad[kMaxProblems];
-    double fin[kMaxProblems][2];
+// What every block needs for the next step.
+struct PlanEntry {
+    int flags;
+    float alpha, beta;
+    int tile_off;   // prefix of tiles over problems doing an A-step
+    int chunk_off;  // prefix of node chunks over problems with node work
+    int pad[3];
 };
 
-struct Barrier {
-    unsigned count;
+struct Workspace {
+    unsigned long long hi[kMaxProblems][kSlots];
+    unsigned long long lo[kMaxProblems][kSlots];
+    int bad[kMaxProblems];
+    PlanEntry plan[kMaxProblems];
+    int n_tiles_active, n_chunks_active, any_active, pad0;
+    unsigned group_cnt[kMaxGroups];
+    unsigned root_cnt;
     unsigned gen;
     int err;
-    int pad;
+    int pad1;
 };
 
-// ---------------------------------------------------------------------------
-// Grid barrier with a bounded wait: on timeout (or when another block timed
-// out) it returns false and the caller leaves its loop, so every wave exits.
-// ---------------------------------------------------------------------------
-__device__ bool grid_sync(Barrier* b, unsigned nblocks, uint64_t timeout_ticks) {
-    __shared__ int s_ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        unsigned g = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned arrived = atomicAdd(&b->count, 1u);
-        if (arrived == nblocks - 1) {
-            atomicExch(&b->count, 0u);
-            __threadfence();
-            atomicAdd(&b->gen, 1u);
-        } else {
-            const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                if (__hip_atomic_load(&b->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    ok = 0;
-                    break;
-                }
-                if (wall_clock64() - t0 > timeout_ticks) {
-                    atomicExch(&b->err, 1);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        __threadfence();
-        s_ok = ok;
-    }
-    __syncthreads();
-    return s_ok != 0;
-}
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float2 ld2(const float* p, int n) { return reinterpret_cast<const float2*>(p)[n]; }
-__device__ __forceinline__ void st2(float* p, int n, float2 v) { reinterpret_cast<float2*>(p)[n] = v; }
+__device__ __forceinline__ float2 ld2(const float* p, int n) {
+    const f2v v = gld(reinterpret_cast<const f2v*>(p))[n];
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(float* p, int n, float2 v) {
+    f2v w;
+    w.x = v.x;
+    w.y = v.y;
+    gst(reinterpret_cast<f2v*>(p))[n] = w;
+}
 
 // Search direction d = -g (restart) or -g + beta * d_prev, single rounding per
 // component so every place that forms it gets the same bits.
@@ -158,7 +146,7 @@ __device__ __forceinline__ float2 step_to(float2 u, float2 d, float alpha) {
 }
 
 // Exact split of a non-negative double (resolution 2^-40) into integer and
-// fractional fixed-point words.
+// fractional fixed-point words; integer sums do not depend on their order.
 __device__ __forceinline__ void fix_split(double v, unsigned long long& hi, unsigned long long& lo) {
     double ip = floor(v);
     double fr = (v - ip) * kFixScale;
@@ -166,6 +154,7 @@ __device__ __forceinline__ void fix_split(double v, unsigned long long& hi, unsi
     lo += (unsigned long long)fr;
 }
 
+// largest p with pref[p] <= item (pref non-decreasing, pref[0] = 0)
 __device__ __forceinline__ int find_prob(const int* pref, int P, int item) {
     int lo = 0, hi = P - 1;
     while (lo < hi) {
@@ -178,89 +167,196 @@ __device__ __forceinline__ int find_prob(const int* pref, int P, int item) {
     return lo;
 }
 
-// Last node chunk of a problem to finish reduces the chunk partials in fixed order.
-__device__ void chunk_done(double v0, double v1, double* NCP, int c, int nchunks, Accum* acc, int p, int lane) {
-    if (lane == 0) {
-        NCP[2 * c] = v0;
-        NCP[2 * c + 1] = v1;
-    }
-    __threadfence();
-    unsigned old = 0;
-    if (lane == 0) old = atomicAdd(&acc->cnt[p], 1u);
-    old = __shfl(old, 0, kWave);
-    if (old == (unsigned)(nchunks - 1)) {
-        __threadfence();
-        double s0 = 0.0, s1 = 0.0;
-        for (int k = lane; k < nchunks; k += kWave) {
-            s0 += __hip_atomic_load(&NCP[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s1 += __hip_atomic_load(&NCP[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        if (lane == 0) {
-            acc->fin[p][0] = s0;
-            acc->fin[p][1] = s1;
-        }
-    }
+// Per-tile record (32 floats, written by the B-step scatter and at start):
+// [a*6 + 0..5] = u, g, d_prev of local node a (a = 0..3), [24] = pixel count,
+// [25..28] = floor(min x) - 1, floor(max x), floor(min y) - 1, floor(max y)
+// over the tile's pixels (int bits).
+constexpr int kTV = 32;
+constexpr int kLanesPerNode = 8;
+constexpr int kNodesPerWave = kWave / kLanesPerNode;
+constexpr int kNodesPerItem = kNodesPerWave * kWavesPerBlock;  // node item = one block
+constexpr int kBoxMax = 512;  // floats of staged image per wave (8 loads per lane)
+
+struct TilePix {  // per-lane pixel data of one tile
+    float x, y, f;
+    float4 N;
+};
+
+__device__ __forceinline__ float tile_tv(const ProbDesc* __restrict__ D, int t, int lane) {
+    return lane < 29 ? gld(D->TV)[(size_t)t * kTV + lane] : 0.f;
 }
 
-// A-step, one tile of <= 64 pixels per wave.
-__device__ void tile_pass(const ProbDesc* __restrict__ D, int t, int lane, int flags, float alpha, float beta,
-                          unsigned long long& hi, unsigned long long& lo, int& bad) {
-    const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
-    const bool restart = (flags & F_RESTART) != 0;
-    const int4 hdr = D->tile_hdr[t];
-    const int4 nd = D->tile_nodes[t];
-    const float* U = D->U[cur];
-    const float* G = D->G[cur];
-    const float* Dp = D->D[dcur];
-    const int nid[4] = {nd.x, nd.y, nd.z, nd.w};
+__device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int t, int lane) {
+    TilePix r;
+    const size_t i = (size_t)t * kTile + lane;
+    r.x = gld(D->px)[i];
+    r.y = gld(D->py)[i];
+    const f4v nv = gld(reinterpret_cast<const f4v*>(D->N4))[i];
+    r.N = make_float4(nv.x, nv.y, nv.z, nv.w);
+    r.f = gld(D->F)[i];
+    return r;
+}
+
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Candidate node displacements of a tile (wave-uniform) and the box of image
+// rows/columns its samples can touch: static pixel bbox widened by the node
+// displacement range (+1 px margin each side) and the cubic support.
+struct TileGeo {
     float2 uc[4];
+    int cnt, bx0, by0, bw, bh;
+    bool staged;
+};
+
+__device__ __forceinline__ TileGeo tile_geo(float tv, int flags, float alpha, float beta) {
+    const bool restart = (flags & F_RESTART) != 0;
+    TileGeo g;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        const int n = nid[a];
-        const float2 d = direction(ld2(G, n), ld2(Dp, n), beta, restart);
-        uc[a] = step_to(ld2(U, n), d, alpha);
+        const float2 u = make_float2(bcast(tv, 6 * a), bcast(tv, 6 * a + 1));
+        const float2 gr = make_float2(bcast(tv, 6 * a + 2), bcast(tv, 6 * a + 3));
+        const float2 dp = make_float2(bcast(tv, 6 * a + 4), bcast(tv, 6 * a + 5));
+        g.uc[a] = step_to(u, direction(gr, dp, beta, restart), alpha);
     }
-    const bool valid = lane < hdr.y;
-    float r = 0.f, gx = 0.f, gy = 0.f;
-    float4 N = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid) {
-        const int i = hdr.x + lane;
-        const float x = D->px[i], y = D->py[i];
-        N = D->N4[i];
-        const float f = D->F[i];
-        const float ux = N.x * uc[0].x + N.y * uc[1].x + N.z * uc[2].x + N.w * uc[3].x;
-        const float uy = N.x * uc[0].y + N.y * uc[1].y + N.z * uc[2].y + N.w * uc[3].y;
-        float gv;
-        sample(D->interp, D->img, D->H, D->W, x + ux, y + uy, gv, gx, gy);
-        r = gv - f;
+    g.cnt = __float_as_int(bcast(tv, 24));
+    const float uxmin = fminf(fminf(g.uc[0].x, g.uc[1].x), fminf(g.uc[2].x, g.uc[3].x));
+    const float uxmax = fmaxf(fmaxf(g.uc[0].x, g.uc[1].x), fmaxf(g.uc[2].x, g.uc[3].x));
+    const float uymin = fminf(fminf(g.uc[0].y, g.uc[1].y), fminf(g.uc[2].y, g.uc[3].y));
+    const float uymax = fmaxf(fmaxf(g.uc[0].y, g.uc[1].y), fmaxf(g.uc[2].y, g.uc[3].y));
+    g.staged = (uxmax - uxmin) < 64.f && (uymax - uymin) < 64.f && fabsf(uxmin) < 1e6f && fabsf(uymin) < 1e6f;
+    g.bx0 = g.by0 = g.bw = g.bh = 0;
+    if (g.staged) {
+        g.bx0 = __float_as_int(bcast(tv, 25)) + (int)floorf(uxmin) - 2;
+        const int bx1 = __float_as_int(bcast(tv, 26)) + (int)floorf(uxmax) + 3;
+        g.by0 = __float_as_int(bcast(tv, 27)) + (int)floorf(uymin) - 2;
+        const int by1 = __float_as_int(bcast(tv, 28)) + (int)floorf(uymax) + 3;
+        g.bw = bx1 - g.bx0 + 1;
+        g.bh = by1 - g.by0 + 1;
+        g.staged = g.bw * g.bh <= kBoxMax;
     }
-    const double r2 = wave_sum((double)r * (double)r);
-    if (lane == 0) {
-        if (!(r2 <= 1e300))
-            bad = 1;
-        else
-            fix_split(r2, hi, lo);
-    }
-    const float qx = r * gx, qy = r * gy;
-    float s0 = wave_sum(qx * N.x), s1 = wave_sum(qy * N.x);
-    float s2 = wave_sum(qx * N.y), s3 = wave_sum(qy * N.y);
-    float s4 = wave_sum(qx * N.z), s5 = wave_sum(qy * N.z);
-    float s6 = wave_sum(qx * N.w), s7 = wave_sum(qy * N.w);
-    if (lane == 0) {
-        float4* o = reinterpret_cast<float4*>(D->TP + 8 * (size_t)t);
-        o[0] = make_float4(s0, s1, s2, s3);
-        o[1] = make_float4(s4, s5, s6, s7);
+    return g;
+}
+
+constexpr int kBoxRegs = kBoxMax / kWave;
+
+// Loads of the staged box into registers (issued one tile ahead of use).
+__device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const TileGeo& g, int lane,
+                                         float (&v)[kBoxRegs]) {
+    if (!g.staged) return;
+    const int n = g.bw * g.bh, interp = D->interp, H = D->H, W = D->W;
+    const D2IC_GLOBAL float* img = gld(D->img);
+    const float inv_bw = 1.0f / (float)g.bw;
+#pragma unroll
+    for (int j = 0; j < kBoxRegs; ++j) {
+        if (j * kWave < n) {
+            const int k = j * kWave + lane;
+            const int kk = k < n ? k : n - 1;
+            const int r = (int)(((float)kk + 0.5f) * inv_bw);
+            const int c = kk - r * g.bw;
+            v[j] = img[(size_t)clamp_row(interp, g.by0 + r, H, W) * W + clamp_col(interp, g.bx0 + c, H, W)];
+        }
     }
 }
 
-// A-step, one chunk of 64 nodes per wave: d, g.d and the spring energy at the candidate.
-__device__ void node_pass_a(const ProbDesc* __restrict__ D, int c, int lane, int flags, float alpha, float beta,
-                            Accum* acc, int p) {
+__device__ __forceinline__ void box_store(const TileGeo& g, int lane, const float (&v)[kBoxRegs],
+                                          __attribute__((address_space(3))) float* box) {
+    if (!g.staged) return;
+    const int n = g.bw * g.bh;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < kBoxRegs; ++j) {
+        const int k = j * kWave + lane;
+        if (j * kWave < n && k < n) box[k] = v[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A-step on one tile: residual at the candidate, its sum of squares (lane 63)
+// and the tile's 8 gradient partials.  Samples inside the staged box read LDS;
+// others read global memory (same values).
+__device__ __forceinline__ double tile_compute(const ProbDesc* __restrict__ D, int t, const TilePix& px,
+                                               const TileGeo& g, int lane,
+                                               const __attribute__((address_space(3))) float* box) {
+    const float4 N = px.N;
+    const float ux = N.x * g.uc[0].x + N.y * g.uc[1].x + N.z * g.uc[2].x + N.w * g.uc[3].x;
+    const float uy = N.x * g.uc[0].y + N.y * g.uc[1].y + N.z * g.uc[2].y + N.w * g.uc[3].y;
+    int x0, y0;
+    float fx, fy;
+    split_coord(px.x, ux, x0, fx);
+    split_coord(px.y, uy, y0, fy);
+    const int lx = x0 - g.bx0, ly = y0 - g.by0;
+    const bool in_box = g.staged && lx >= 1 && ly >= 1 && lx + 2 < g.bw && ly + 2 < g.bh;
+    float gv, gx, gy;
+    if (in_box)
+        sample_box(D->interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
+    else
+        sample_at(D->interp, D->img, D->H, D->W, x0, fx, y0, fy, gv, gx, gy);
+    const float r = lane < g.cnt ? gv - px.f : 0.f;
+    const double r2 = wave_sum63((double)r * (double)r);
+    const float qx = r * gx, qy = r * gy;
+    const float s0 = wave_sum63(qx * N.x), s1 = wave_sum63(qy * N.x);
+    const float s2 = wave_sum63(qx * N.y), s3 = wave_sum63(qy * N.y);
+    const float s4 = wave_sum63(qx * N.z), s5 = wave_sum63(qy * N.z);
+    const float s6 = wave_sum63(qx * N.w), s7 = wave_sum63(qy * N.w);
+    if (lane == kWave - 1) {
+        D2IC_GLOBAL f4v* o = gst(reinterpret_cast<f4v*>(D->TP + 8 * (size_t)t));
+        f4v a0, a1;
+        a0.x = s0; a0.y = s1; a0.z = s2; a0.w = s3;
+        a1.x = s4; a1.y = s5; a1.z = s6; a1.w = s7;
+        o[0] = a0;
+        o[1] = a1;
+    }
+    return r2;
+}
+
+// sum over the 8 lanes of an aligned lane group (DPP quad_perm and
+// row_half_mirror); every lane of the group receives the total
+__device__ __forceinline__ double group_sum(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    return v;
+}
+__device__ __forceinline__ float group_sum(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    return v;
+}
+
+// Block-level sum of two per-lane values in fixed order; thread 0 stores them.
+__device__ __forceinline__ void block_store2(double a, double b, double* s_red, double* out) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    a = wave_sum63(a);
+    b = wave_sum63(b);
+    if (lane == kWave - 1) {
+        s_red[2 * wave] = a;
+        s_red[2 * wave + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            x += s_red[2 * w];
+            y += s_red[2 * w + 1];
+        }
+        gst(out)[0] = x;
+        gst(out)[1] = y;
+    }
+    __syncthreads();
+}
+
+// A-step node item (128 nodes, 8 lanes per node): materialise d, g.d and the
+// spring energy at the candidate (both halves of every ordered pair).
+__device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
+                            double* s_red) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
     const bool restart = (flags & F_RESTART) != 0;
-    const int n = c * kWave + lane;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int sub = lane % kLanesPerNode;
+    const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
     double gd = 0.0, E = 0.0;
     if (n < D->nnodes) {
         const float* U = D->U[cur];
@@ -268,250 +364,437 @@ __device__ void node_pass_a(const ProbDesc* __restrict__ D, int c, int lane, int
         const float* Dp = D->D[dcur];
         const float2 g = ld2(G, n);
         const float2 d = direction(g, ld2(Dp, n), beta, restart);
-        st2(D->D[dcur ^ 1], n, d);
-        gd = (double)g.x * (double)d.x + (double)g.y * (double)d.y;
+        if (sub == 0) {
+            st2(D->D[dcur ^ 1], n, d);
+            gd = (double)g.x * (double)d.x + (double)g.y * (double)d.y;
+        }
         const float2 ucn = step_to(ld2(U, n), d, alpha);
         const int md = D->nb_maxdeg;
-        for (int k = 0; k < md; ++k) {
-            const int j = D->nb_idx[(size_t)n * md + k];
-            if (j < 0) break;
-            const float w = D->nb_wp[(size_t)n * md + k];
-            const float2 dj = direction(ld2(G, j), ld2(Dp, j), beta, restart);
-            const float2 ucj = step_to(ld2(U, j), dj, alpha);
+        for (int k = sub; k < md; k += kLanesPerNode) {
+            const int j = gld(D->nb_idx)[(size_t)n * md + k];
+            if (j < 0) continue;
+            const float w = gld(D->nb_wp)[(size_t)n * md + k];
+            const float2 ucj = step_to(ld2(U, j), direction(ld2(G, j), ld2(Dp, j), beta, restart), alpha);
             const float dx = ucn.x - ucj.x, dy = ucn.y - ucj.y;
             E += (double)(w * (dx * dx + dy * dy));
         }
     }
-    gd = wave_sum(gd);
-    E = wave_sum(E);
-    chunk_done(gd, E, D->NCP, c, D->nchunks, acc, p, lane);
+    block_store2(gd, E, s_red, D->NCP + 2 * (size_t)item);
 }
 
-// B-step, one chunk of 64 nodes per wave: new point, gradient gather, dots.
-__device__ void node_pass_b(const ProbDesc* __restrict__ D, int c, int lane, int flags, float alpha, Accum* acc,
-                            int p) {
+// B-step node item: new point, gradient gather, |g|^2, g.(g - g_prev), and
+// the scatter of (u, g, d) into the tile records for the next A-step.
+__device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, double* s_red) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
-    const int n = c * kWave + lane;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int sub = lane % kLanesPerNode;
+    const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
+    const bool valid = n < D->nnodes;
     double gg = 0.0, gnum = 0.0;
-    if (n < D->nnodes) {
-        const float* U = D->U[cur];
-        const float* Dn = D->D[dcur ^ 1];
-        const float2 un = step_to(ld2(U, n), ld2(Dn, n), alpha);
-        st2(D->U[cur ^ 1], n, un);
-        double sx = 0.0, sy = 0.0;
-        const int e0 = D->node_ptr[n], e1 = D->node_ptr[n + 1];
-        for (int e = e0; e < e1; ++e) {
-            const float2 q = ld2(D->TP, D->node_ent[e]);
+    float2 un = make_float2(0.f, 0.f), dn = make_float2(0.f, 0.f), go = make_float2(0.f, 0.f);
+    double sx = 0.0, sy = 0.0;
+    float rx = 0.f, ry = 0.f;
+    int e0 = 0, e1 = 0;
+    const float* U = D->U[cur];
+    const float* Dn = D->D[dcur ^ 1];
+    if (valid) {
+        dn = ld2(Dn, n);
+        un = step_to(ld2(U, n), dn, alpha);
+        go = ld2(D->G[cur], n);
+        e0 = gld(D->node_ptr)[n];
+        e1 = gld(D->node_ptr)[n + 1];
+        for (int e = e0 + sub; e < e1; e += kLanesPerNode) {
+            const float2 q = ld2(D->TP, gld(D->node_ent)[e]);
             sx += (double)q.x;
             sy += (double)q.y;
         }
-        float rx = 0.f, ry = 0.f;
         const int md = D->nb_maxdeg;
-        for (int k = 0; k < md; ++k) {
-            const int j = D->nb_idx[(size_t)n * md + k];
-            if (j < 0) break;
-            const float w = D->nb_wp[(size_t)n * md + k];
+        for (int k = sub; k < md; k += kLanesPerNode) {
+            const int j = gld(D->nb_idx)[(size_t)n * md + k];
+            if (j < 0) continue;
+            const float w = gld(D->nb_wp)[(size_t)n * md + k];
             const float2 uj = step_to(ld2(U, j), ld2(Dn, j), alpha);
             rx += w * (un.x - uj.x);
             ry += w * (un.y - uj.y);
         }
+    }
+    sx = group_sum(sx);
+    sy = group_sum(sy);
+    rx = group_sum(rx);
+    ry = group_sum(ry);
+    if (valid) {
         const float gx = (float)(sx * D->invNp) + D->alpha_reg * rx;
         const float gy = (float)(sy * D->invNp) + D->alpha_reg * ry;
-        const float2 go = ld2(D->G[cur], n);
-        st2(D->G[cur ^ 1], n, make_float2(gx, gy));
-        gg = (double)gx * (double)gx + (double)gy * (double)gy;
-        gnum = (double)gx * ((double)gx - (double)go.x) + (double)gy * ((double)gy - (double)go.y);
+        if (sub == 0) {
+            st2(D->U[cur ^ 1], n, un);
+            st2(D->G[cur ^ 1], n, make_float2(gx, gy));
+            gg = (double)gx * (double)gx + (double)gy * (double)gy;
+            gnum = (double)gx * ((double)gx - (double)go.x) + (double)gy * ((double)gy - (double)go.y);
+        }
+        for (int e = e0 + sub; e < e1; e += kLanesPerNode) {
+            const int ent = gld(D->node_ent)[e];
+            float* rec = D->TV + (size_t)(ent >> 2) * kTV + 6 * (ent & 3);
+            st2(rec, 0, un);
+            st2(rec, 1, make_float2(gx, gy));
+            st2(rec, 2, dn);
+        }
     }
-    gg = wave_sum(gg);
-    gnum = wave_sum(gnum);
-    chunk_done(gg, gnum, D->NCP, c, D->nchunks, acc, p, lane);
+    block_store2(gg, gnum, s_red, D->NCP + 2 * (size_t)item);
 }
 
+// State transition of one problem after a step (root block only).  just_did_a:
+// the finished step was an A-step (Armijo decision follows), else a B-step
+// (stop test and beta follow).  fin: node-chunk sums of the finished step.
+__device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double sr2, bool bad, double fin0,
+                      double fin1) {
+    int flags = 0;
+    if (st.mode == MODE_DONE) return 0;
+    if (just_did_a) {
+        // B-decision: Armijo test on the trial just evaluated (solver_global_cg.py:419-448)
+        if (st.mode == MODE_PASS || st.mode == MODE_INIT) {
+            const double gd = fin0, E = fin1;
+            double Jc = 0.5 * sr2 * D.invNp + (double)D.alpha_reg * (0.25 * E);
+            if (bad) Jc = NAN;
+            bool accept = false;
+            if (st.mode == MODE_INIT || st.final_eval) {
+                accept = true;
+            } else if (st.trial == 0 && !st.restart && gd >= 0.0) {
+                st.restart = 1;  // non-descent direction: redo the pass with d = -g
+            } else if (Jc <= st.J + 1e-4 * (double)st.alpha * gd) {
+                accept = true;
+            } else {
+                st.Jtrial = (float)Jc;
+                st.trial += 1;
+                st.alpha *= 0.5f;
+                if (st.trial == 10) st.final_eval = 1;  // alpha = 2^-10 is taken untested
+            }
+            if (accept) {
+                flags |= F_DOB;
+                if (st.mode != MODE_INIT) {
+                    if (D.hist) {
+                        D.hist[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
+                        D.hist[3 * st.k + 1] = st.gnorm;
+                        D.hist[3 * st.k + 2] = st.alpha;
+                    }
+                    st.k += 1;
+                }
+                st.J = Jc;
+                st.mode = MODE_NODE;
+            }
+        }
+    } else {
+        // A-decision: stop test on |grad| at the new point, PR+ beta (solver_global_cg.py:369-415)
+        if (st.mode == MODE_NODE) {
+            const double gg = fin0, gnum = fin1;
+            st.cur ^= 1;
+            st.dcur ^= 1;
+            const float gnorm = (float)sqrt(gg);
+            if (st.k >= D.max_iter) {
+                // while_loop exits on k < max_iter before evaluating anything
+                st.mode = MODE_DONE;
+                *D.n_iters = st.k;
+            } else if (gnorm < D.tol) {
+                st.mode = MODE_DONE;
+                if (D.hist) {
+                    D.hist[3 * st.k] = (float)st.J;
+                    D.hist[3 * st.k + 1] = gnorm;
+                    D.hist[3 * st.k + 2] = 0.f;
+                }
+                *D.n_iters = st.k;
+            } else {
+                float b = 0.f;
+                if (st.k > 0) {
+                    b = (float)gnum / ((float)st.gg + 1e-16f);
+                    if (b < 0.f) b = 0.f;  // max(beta, 0); NaN propagates like jnp.maximum
+                }
+                st.beta = b;
+                st.gg = gg;
+                st.gnorm = gnorm;
+                st.mode = MODE_PASS;
+                st.alpha = 1.f;
+                st.trial = 0;
+                st.restart = 0;
+                st.final_eval = 0;
+            }
+        }
+        if (st.mode == MODE_PASS || st.mode == MODE_INIT) {
+            flags |= F_DOA;
+            st.passes += 1;
+        }
+    }
+    if (st.restart) flags |= F_RESTART;
+    if (st.cur) flags |= F_CUR;
+    if (st.dcur) flags |= F_DCUR;
+    return flags;
+}
+
+// Root block: reduce the finished step, decide every problem, publish the plan.
+__device__ void root_step(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S, Workspace* ws,
+                          bool just_did_a, const int* s_flags, double (*s_fin)[2], double* s_sr2) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    // node-chunk sums in fixed order, one wave per problem
+    for (int p = wave; p < P; p += kWavesPerBlock) {
+        const int fl = s_flags[p];
+        const bool had = just_did_a ? (fl & F_DOA) : (fl & F_DOB);
+        double a = 0.0, b = 0.0;
+        if (had) {
+            const ProbDesc& D = descs[p];
+            for (int c = lane; c < D.nchunks; c += kWave) {
+                a += __hip_atomic_load(&D.NCP[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                b += __hip_atomic_load(&D.NCP[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        if (lane == 0) {
+            s_fin[p][0] = a;
+            s_fin[p][1] = b;
+        }
+    }
+    for (int p = threadIdx.x; p < P; p += kBlock) {
+        unsigned long long hi = 0, lo = 0;
+        for (int k = 0; k < kSlots; ++k) {
+            hi += __hip_atomic_load(&ws->hi[p][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lo += __hip_atomic_load(&ws->lo[p][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ws->hi[p][k] = 0ull;
+            ws->lo[p][k] = 0ull;
+        }
+        s_sr2[p] = (double)hi + (double)lo / kFixScale;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < P; p += kBlock) {
+        ProbState st = S[p];
+        const int bad = ws->bad[p];
+        ws->bad[p] = 0;
+        const int fl = decide(descs[p], st, just_did_a, s_sr2[p], bad != 0, s_fin[p][0], s_fin[p][1]);
+        S[p] = st;
+        ws->plan[p].flags = fl;
+        ws->plan[p].alpha = st.alpha;
+        ws->plan[p].beta = st.beta;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int to = 0, co = 0, any = 0;
+        for (int p = 0; p < P; ++p) {
+            const int fl = ws->plan[p].flags;
+            ws->plan[p].tile_off = to;
+            ws->plan[p].chunk_off = co;
+            if (fl & F_DOA) to += descs[p].ntiles;
+            if (fl & (F_DOA | F_DOB)) co += descs[p].nchunks;
+            if (S[p].mode != MODE_DONE) any = 1;  // a problem whose trial was rejected idles one step
+        }
+        ws->n_tiles_active = to;
+        ws->n_chunks_active = co;
+        ws->any_active = any;
+    }
+}
+
+// wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
+// a wait is abandoned only after both 2^20 polls and 4 s.
+constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
+constexpr unsigned kBarrierMinPolls = 1u << 20;
+
 __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __restrict__ descs, int P,
-                                                          ProbState* __restrict__ S, Accum* __restrict__ acc,
-                                                          Barrier* __restrict__ bar, int max_steps,
-                                                          uint64_t timeout_ticks) {
+                                                          ProbState* __restrict__ S, Workspace* __restrict__ ws,
+                                                          int max_steps) {
     __shared__ int s_flags[kMaxProblems];
     __shared__ float s_alpha[kMaxProblems];
     __shared__ float s_beta[kMaxProblems];
-    __shared__ int s_itemA[kMaxProblems + 1];
-    __shared__ int s_itemB[kMaxProblems + 1];
-    __shared__ int s_any;
+    __shared__ int s_toff[kMaxProblems];
+    __shared__ int s_coff[kMaxProblems];
+    __shared__ unsigned long long s_hi[kMaxProblems];
+    __shared__ unsigned long long s_lo[kMaxProblems];
+    __shared__ int s_bad[kMaxProblems];
+    __shared__ double s_fin[kMaxProblems][2];
+    __shared__ double s_sr2[kMaxProblems];
+    __shared__ int s_hdr[3];
+    __shared__ int s_root, s_ok;
+    __shared__ double s_red[2 * kWavesPerBlock];
+    __shared__ float s_box[kWavesPerBlock][kBoxMax];
+    __attribute__((address_space(3))) float* my_box = (__attribute__((address_space(3))) float*)s_box[threadIdx.x / kWave];
 
-    if (threadIdx.x == 0) {
-        int a = 0, b = 0;
-        for (int p = 0; p < P; ++p) {
-            s_itemA[p] = a;
-            s_itemB[p] = b;
-            a += descs[p].nchunks + descs[p].ntiles;
-            b += descs[p].nchunks;
-        }
-        s_itemA[P] = a;
-        s_itemB[P] = b;
-    }
-    __syncthreads();
-    const int nA = s_itemA[P], nB = s_itemB[P];
     const int lane = threadIdx.x & (kWave - 1);
-    const int gwave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
-    const int nwaves = gridDim.x * kWavesPerBlock;
-    const bool writer = blockIdx.x == 0;
+    const int wave = threadIdx.x / kWave;
+    const int nb = gridDim.x;
+    // XCD-aware numbering: consecutive blocks sit on different XCDs, so give each
+    // XCD one contiguous share of the tiles (and of the image).
+    const int vb = (nb % 8 == 0) ? (int)((blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8) : (int)blockIdx.x;
+    const int gwave = __builtin_amdgcn_readfirstlane(vb * kWavesPerBlock + wave);
+    const int nwaves = nb * kWavesPerBlock;
+    const int group = blockIdx.x / kGroup;
+    const int ngroups = (nb + kGroup - 1) / kGroup;
+    const unsigned gsize = (unsigned)min(kGroup, nb - group * kGroup);
 
     for (int step = 0; step < max_steps; ++step) {
         const bool isA = (step & 1) == 0;
-        const ProbState* Sin = S + (size_t)(step & 1) * P;
-        ProbState* Sout = S + (size_t)((step + 1) & 1) * P;
-        const Accum* aprev = acc + (step + 2) % 3;
-        Accum* acur = acc + step % 3;
-        Accum* azero = acc + (step + 1) % 3;
-        if (threadIdx.x == 0) s_any = 0;
-        __syncthreads();
-
-        // ---- decisions (identical in every block; block 0 records them) ----
+        // ---- read the plan published for this step ----
         for (int p = threadIdx.x; p < P; p += kBlock) {
-            const ProbDesc& D = descs[p];
-            ProbState st = Sin[p];
-            int flags = 0;
-            if (isA) {
-                if (st.mode == MODE_NODE) {
-                    const double gg = aprev->fin[p][0], gnum = aprev->fin[p][1];
-                    st.cur ^= 1;
-                    st.dcur ^= 1;
-                    const float gnorm = (float)sqrt(gg);
-                    if (st.k >= D.max_iter) {
-                        // while_loop exits on k < max_iter before evaluating anything
-                        st.mode = MODE_DONE;
-                        if (writer) *D.n_iters = st.k;
-                    } else if (gnorm < D.tol) {
-                        st.mode = MODE_DONE;
-                        if (writer) {
-                            if (D.hist) {
-                                D.hist[3 * st.k] = (float)st.J;
-                                D.hist[3 * st.k + 1] = gnorm;
-                                D.hist[3 * st.k + 2] = 0.f;
-                            }
-                            *D.n_iters = st.k;
-                        }
-                    } else {
-                        float b = 0.f;
-                        if (st.k > 0) {
-                            b = (float)gnum / ((float)st.gg + 1e-16f);
-                            if (b < 0.f) b = 0.f;  // max(beta, 0); NaN propagates like jnp.maximum
-                        }
-                        st.beta = b;
-                        st.gg = gg;
-                        st.gnorm = gnorm;
-                        st.mode = MODE_PASS;
-                        st.alpha = 1.f;
-                        st.trial = 0;
-                        st.restart = 0;
-                        st.final_eval = 0;
-                    }
-                }
-                if (st.mode == MODE_PASS || st.mode == MODE_INIT) {
-                    flags |= F_DOA;
-                    st.passes += 1;
-                }
-            } else {
-                if (st.mode == MODE_PASS || st.mode == MODE_INIT) {
-                    const double sr2 = (double)aprev->hi[p] + (double)aprev->lo[p] / kFixScale;
-                    const double gd = aprev->fin[p][0], E = aprev->fin[p][1];
-                    double Jc = 0.5 * sr2 * D.invNp + (double)D.alpha_reg * (0.25 * E);
-                    if (aprev->bad[p]) Jc = NAN;
-                    bool accept = false;
-                    if (st.mode == MODE_INIT || st.final_eval) {
-                        accept = true;
-                    } else if (st.trial == 0 && !st.restart && gd >= 0.0) {
-                        st.restart = 1;  // non-descent direction: redo the pass with d = -g
-                    } else if (Jc <= st.J + 1e-4 * (double)st.alpha * gd) {
-                        accept = true;
-                    } else {
-                        st.Jtrial = (float)Jc;
-                        st.trial += 1;
-                        st.alpha *= 0.5f;
-                        if (st.trial == 10) st.final_eval = 1;  // alpha = 2^-10 is taken untested
-                    }
-                    if (accept) {
-                        flags |= F_DOB;
-                        if (st.mode != MODE_INIT) {
-                            if (writer && D.hist) {
-                                D.hist[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
-                                D.hist[3 * st.k + 1] = st.gnorm;
-                                D.hist[3 * st.k + 2] = st.alpha;
-                            }
-                            st.k += 1;
-                        }
-                        st.J = Jc;
-                        st.mode = MODE_NODE;
-                    }
-                }
-            }
-            if (st.restart) flags |= F_RESTART;
-            if (st.cur) flags |= F_CUR;
-            if (st.dcur) flags |= F_DCUR;
-            s_flags[p] = flags;
-            s_alpha[p] = st.alpha;
-            s_beta[p] = st.beta;
-            if (flags & (F_DOA | F_DOB)) atomicOr(&s_any, 1);
-            if (writer) {
-                Sout[p] = st;
-                if (st.mode == MODE_DONE) S[2 * (size_t)P + p] = st;
-                azero->fin[p][0] = 0.0;
-                azero->fin[p][1] = 0.0;
-                azero->hi[p] = 0ull;
-                azero->lo[p] = 0ull;
-                azero->cnt[p] = 0u;
-                azero->bad[p] = 0;
-            }
+            const PlanEntry e = ws->plan[p];
+            s_flags[p] = e.flags;
+            s_alpha[p] = e.alpha;
+            s_beta[p] = e.beta;
+            s_toff[p] = e.tile_off;
+            s_coff[p] = e.chunk_off;
+            s_hi[p] = 0ull;
+            s_lo[p] = 0ull;
+            s_bad[p] = 0;
+        }
+        if (threadIdx.x == 0) {
+            s_hdr[0] = ws->n_tiles_active;
+            s_hdr[1] = ws->n_chunks_active;
+            s_hdr[2] = ws->any_active;
         }
         __syncthreads();
-        if (!s_any) break;
+        if (!s_hdr[2]) break;
+        const int nT = s_hdr[0], nC = s_hdr[1];
 
-        // ---- work ----
-        if (isA) {
-            unsigned long long hi = 0ull, lo = 0ull;
-            int bad = 0, accp = -1;
-            for (int it = gwave; it < nA; it += nwaves) {
-                const int p = find_prob(s_itemA, P, it);
-                const int flags = s_flags[p];
-                if (!(flags & F_DOA)) continue;
-                const ProbDesc* D = descs + p;
-                const int l = it - s_itemA[p];
-                const int nch = D->nchunks;
-                if (l < nch) {
-                    node_pass_a(D, l, lane, flags, s_alpha[p], s_beta[p], acur, p);
-                } else {
-                    if (p != accp) {
-                        if (accp >= 0 && lane == 0) {
-                            if (hi) atomicAdd(&acur->hi[accp], hi);
-                            if (lo) atomicAdd(&acur->lo[accp], lo);
-                            if (bad) atomicOr(&acur->bad[accp], 1);
+        // ---- node items, one block each (A: d, g.d, spring energy; B: gradient, dots) ----
+        for (int c = (int)vb; c < nC; c += nb) {
+            const int p = find_prob(s_coff, P, c);
+            const ProbDesc* D = descs + p;
+            const int fl = s_flags[p];
+            if (isA)
+                node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red);
+            else
+                node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red);
+        }
+
+        // ---- tiles (A only): contiguous share per wave, software-pipelined:
+        //      record 2 tiles ahead, pixel data and staged box 1 tile ahead ----
+        if (isA && nT > 0) {
+            const int t0 = (int)(((long long)nT * gwave) / nwaves);
+            const int t1 = (int)(((long long)nT * (gwave + 1)) / nwaves);
+            if (t0 < t1) {
+                // problem of global tile index it (monotone walk)
+                int pa = find_prob(s_toff, P, t0);
+                auto next_prob = [&](int p, int it) {
+                    while (p + 1 < P && it >= s_toff[p + 1]) ++p;
+                    return p;
+                };
+                pa = next_prob(pa, t0);
+                unsigned long long hi = 0ull, lo = 0ull;
+                int bad = 0;
+                // prologue: tile t0 record, geometry, box and pixels; t0+1 record
+                float tv_cur = tile_tv(descs + pa, t0 - s_toff[pa], lane);
+                TileGeo g_cur = tile_geo(tv_cur, s_flags[pa], s_alpha[pa], s_beta[pa]);
+                float R[kBoxRegs];
+                box_load(descs + pa, g_cur, lane, R);
+                TilePix px_cur = tile_pix(descs + pa, t0 - s_toff[pa], lane);
+                int pb = pa;
+                float tv_nxt = 0.f;
+                if (t0 + 1 < t1) {
+                    pb = next_prob(pa, t0 + 1);
+                    tv_nxt = tile_tv(descs + pb, t0 + 1 - s_toff[pb], lane);
+                }
+                for (int it = t0; it < t1; ++it) {
+                    // stage 1: box of this tile into LDS
+                    box_store(g_cur, lane, R, my_box);
+                    // stage 2: next tile's geometry and box loads; record two ahead; next pixels
+                    TileGeo g_nxt;
+                    TilePix px_nxt;
+                    float tv_nn = 0.f;
+                    int pc = pb;
+                    if (it + 1 < t1) {
+                        g_nxt = tile_geo(tv_nxt, s_flags[pb], s_alpha[pb], s_beta[pb]);
+                        box_load(descs + pb, g_nxt, lane, R);
+                        if (it + 2 < t1) {
+                            pc = next_prob(pb, it + 2);
+                            tv_nn = tile_tv(descs + pc, it + 2 - s_toff[pc], lane);
+                        }
+                        px_nxt = tile_pix(descs + pb, it + 1 - s_toff[pb], lane);
+                    }
+                    // stage 3: this tile
+                    const double r2 = tile_compute(descs + pa, it - s_toff[pa], px_cur, g_cur, lane, my_box);
+                    if (lane == kWave - 1) {
+                        if (!(r2 <= 1e300))
+                            bad = 1;
+                        else
+                            fix_split(r2, hi, lo);
+                    }
+                    if (pb != pa || it + 1 >= t1) {
+                        if (lane == kWave - 1) {
+                            atomicAdd(&s_hi[pa], hi);
+                            atomicAdd(&s_lo[pa], lo);
+                            if (bad) atomicOr(&s_bad[pa], 1);
                         }
                         hi = lo = 0ull;
                         bad = 0;
-                        accp = p;
                     }
-                    tile_pass(D, l - nch, lane, flags, s_alpha[p], s_beta[p], hi, lo, bad);
+                    pa = pb;
+                    pb = pc;
+                    g_cur = g_nxt;
+                    px_cur = px_nxt;
+                    tv_nxt = tv_nn;
                 }
             }
-            if (accp >= 0 && lane == 0) {
-                if (hi) atomicAdd(&acur->hi[accp], hi);
-                if (lo) atomicAdd(&acur->lo[accp], lo);
-                if (bad) atomicOr(&acur->bad[accp], 1);
-            }
-        } else {
-            for (int it = gwave; it < nB; it += nwaves) {
-                const int p = find_prob(s_itemB, P, it);
-                const int flags = s_flags[p];
-                if (!(flags & F_DOB)) continue;
-                node_pass_b(descs + p, it - s_itemB[p], lane, flags, s_alpha[p], acur, p);
+        }
+        __syncthreads();
+        if (isA) {
+            const int slot = blockIdx.x % kSlots;
+            for (int p = threadIdx.x; p < P; p += kBlock) {
+                if (s_hi[p]) atomicAdd(&ws->hi[p][slot], s_hi[p]);
+                if (s_lo[p]) atomicAdd(&ws->lo[p][slot], s_lo[p]);
+                if (s_bad[p]) atomicOr(&ws->bad[p], 1);
             }
         }
-        if (!grid_sync(bar, gridDim.x, timeout_ticks)) break;
+
+        // ---- tree barrier; the last block to arrive runs the root step ----
+        __syncthreads();
+        unsigned g = 0;
+        if (threadIdx.x == 0) {
+            g = __hip_atomic_load(&ws->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            int root = 0;
+            const unsigned a = atomicAdd(&ws->group_cnt[group], 1u);
+            if (a == gsize - 1) {
+                atomicExch(&ws->group_cnt[group], 0u);
+                __threadfence();
+                const unsigned r = atomicAdd(&ws->root_cnt, 1u);
+                if (r == (unsigned)ngroups - 1) {
+                    atomicExch(&ws->root_cnt, 0u);
+                    root = 1;
+                }
+            }
+            __threadfence();
+            s_root = root;
+            s_ok = 1;
+        }
+        __syncthreads();
+        if (s_root) {
+            root_step(descs, P, S, ws, isA, s_flags, s_fin, s_sr2);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __threadfence();
+                atomicAdd(&ws->gen, 1u);
+            }
+        } else if (threadIdx.x == 0) {
+            const uint64_t tstart = wall_clock64();
+            unsigned polls = 0;
+            while (__hip_atomic_load(&ws->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    s_ok = 0;
+                    break;
+                }
+                ++polls;
+                if (polls > kBarrierMinPolls && wall_clock64() - tstart > kBarrierTimeoutTicks) {
+                    atomicExch(&ws->err, 1);
+                    printf("d2ic: grid barrier timeout (block %u, step %d)\n", blockIdx.x, step);
+                    s_ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __threadfence();
+        }
+        __syncthreads();
+        if (!s_ok) break;
     }
 }
 
-// Per-problem start state, zeroed g/d buffers, NaN-filled history.
-__global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S) {
+// Per-problem start state, first plan (an A-step at alpha = 0, i.e. the value
+// and gradient at u0), zeroed g/d buffers, NaN-filled history.
+__global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S,
+                       Workspace* __restrict__ ws) {
     const int p = blockIdx.y;
     if (p >= P) return;
     const ProbDesc& D = descs[p];
@@ -522,10 +805,21 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
     if (D.hist)
         for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 3 * D.max_iter; k += gridDim.x * blockDim.x)
             D.hist[k] = NAN;
+    // tile records for the first pass: (u0, g = 0, d = 0) per tile node
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 4 * D.ntiles; e += gridDim.x * blockDim.x) {
+        const int t = e >> 2, a = e & 3;
+        const int4 nd = D.tile_nodes[t];
+        const int n = a == 0 ? nd.x : (a == 1 ? nd.y : (a == 2 ? nd.z : nd.w));
+        float* rec = D.TV + (size_t)t * kTV + 6 * a;
+        rec[0] = D.U[0][2 * n];
+        rec[1] = D.U[0][2 * n + 1];
+        rec[2] = rec[3] = rec[4] = rec[5] = 0.f;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ProbState st;
         st.mode = MODE_INIT;
-        st.k = st.trial = st.restart = st.final_eval = st.cur = st.dcur = st.passes = 0;
+        st.k = st.trial = st.restart = st.final_eval = st.cur = st.dcur = 0;
+        st.passes = 1;
         st.alpha = 0.f;
         st.beta = 0.f;
         st.gnorm = 0.f;
@@ -533,30 +827,46 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
         st.J = 0.0;
         st.gg = 0.0;
         S[p] = st;
-        S[P + p] = st;
-        S[2 * P + p] = st;
         *D.n_iters = -1;
+        int to = 0, co = 0;
+        for (int q = 0; q < p; ++q) {
+            to += descs[q].ntiles;
+            co += descs[q].nchunks;
+        }
+        PlanEntry e;
+        e.flags = F_DOA;
+        e.alpha = 0.f;
+        e.beta = 0.f;
+        e.tile_off = to;
+        e.chunk_off = co;
+        e.pad[0] = e.pad[1] = e.pad[2] = 0;
+        ws->plan[p] = e;
+        if (p == P - 1) {
+            ws->n_tiles_active = to + D.ntiles;
+            ws->n_chunks_active = co + D.nchunks;
+            ws->any_active = 1;
+        }
     }
 }
 
 // Copy the final iterate into the caller's buffer (U[0]) and publish counters;
 // n_iters = -1 marks a problem that did not finish (barrier timeout).
 __global__ void k_finalize(const ProbDesc* __restrict__ descs, int P, const ProbState* __restrict__ S,
-                           const Barrier* __restrict__ bar) {
+                           const Workspace* __restrict__ ws) {
     const int p = blockIdx.y;
     if (p >= P) return;
     const ProbDesc& D = descs[p];
-    const ProbState st = S[2 * (size_t)P + p];  // state recorded when the problem finished
+    const ProbState st = S[p];
     if (st.mode == MODE_DONE && st.cur == 1)
         for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * D.nnodes; n += gridDim.x * blockDim.x)
             D.U[0][n] = D.U[1][n];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (D.pass_count) *D.pass_count = st.passes;
-        if (st.mode != MODE_DONE || bar->err) *D.n_iters = -1;
+        if (st.mode != MODE_DONE || ws->err) *D.n_iters = -1;
     }
 }
 
-// f(x_p) sampled once per reference image.
+// f(x_p) sampled once per reference image (padded slots included, harmless).
 __global__ void k_sample_ref(const float* __restrict__ img, int H, int W, int interp, const float* __restrict__ px,
                              const float* __restrict__ py, int64_t n, float* __restrict__ F) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -613,7 +923,7 @@ __global__ void k_group_ntiles(const int32_t* __restrict__ gstart, int64_t ng, i
 }
 
 __global__ void k_tiles(const int32_t* __restrict__ gstart, const int32_t* __restrict__ tstart, int64_t ng, int64_t np,
-                        const int32_t* __restrict__ perm, const int32_t* __restrict__ pnodes, int4* __restrict__ hdr,
+                        const int32_t* __restrict__ perm, const int32_t* __restrict__ pnodes, int32_t* __restrict__ tcnt,
                         int4* __restrict__ tn) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
@@ -624,20 +934,57 @@ __global__ void k_tiles(const int32_t* __restrict__ gstart, const int32_t* __res
                            pnodes[4 * (int64_t)p0 + 3]);
     int t = tstart[g];
     for (int s = s0; s < s1; s += kTile, ++t) {
-        hdr[t] = make_int4(s, min(kTile, s1 - s), (int)g, 0);
+        tcnt[t] = min(kTile, s1 - s);
         tn[t] = nodes;
     }
 }
 
-__global__ void k_soa(const int32_t* __restrict__ perm, int64_t np, const float* __restrict__ coords,
-                      const float* __restrict__ shapeN, float* __restrict__ px, float* __restrict__ py,
-                      float4* __restrict__ N4) {
+// Tile records start zeroed with the pixel count in slot 24 and the static
+// pixel bbox in slots 25..28 (see the record layout).
+__global__ void k_tv_static(const int32_t* __restrict__ tcnt, int64_t nt, const float* __restrict__ px,
+                            const float* __restrict__ py, float* __restrict__ TV) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    for (int k = 0; k < kTV; ++k) TV[t * kTV + k] = 0.f;
+    const int cnt = tcnt[t];
+    float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (int l = 0; l < cnt; ++l) {
+        const float x = px[t * kTile + l], y = py[t * kTile + l];
+        xmin = fminf(xmin, x);
+        xmax = fmaxf(xmax, x);
+        ymin = fminf(ymin, y);
+        ymax = fmaxf(ymax, y);
+    }
+    TV[t * kTV + 24] = __int_as_float(cnt);
+    TV[t * kTV + 25] = __int_as_float((int)floorf(xmin) - 1);
+    TV[t * kTV + 26] = __int_as_float((int)floorf(xmax));
+    TV[t * kTV + 27] = __int_as_float((int)floorf(ymin) - 1);
+    TV[t * kTV + 28] = __int_as_float((int)floorf(ymax));
+}
+
+// Padded layout defaults: pixel centre (0.5, 0.5), zero shape functions.
+__global__ void k_pad_init(int64_t n, float* __restrict__ px, float* __restrict__ py, float4* __restrict__ N4) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    px[s] = 0.5f;
+    py[s] = 0.5f;
+    N4[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// sorted pixel s of group g sits at slot (tstart[g] + q / 64) * 64 + q % 64, q = s - gstart[g]
+__global__ void k_soa(const int32_t* __restrict__ perm, const int32_t* __restrict__ gid,
+                      const int32_t* __restrict__ gstart, const int32_t* __restrict__ tstart, int64_t np,
+                      const float* __restrict__ coords, const float* __restrict__ shapeN, float* __restrict__ px,
+                      float* __restrict__ py, float4* __restrict__ N4) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= np) return;
-    int64_t p = perm[s];
-    px[s] = coords[2 * p];
-    py[s] = coords[2 * p + 1];
-    N4[s] = make_float4(shapeN[4 * p], shapeN[4 * p + 1], shapeN[4 * p + 2], shapeN[4 * p + 3]);
+    const int g = gid[s] - 1;
+    const int q = (int)s - gstart[g];
+    const int64_t o = (int64_t)(tstart[g] + q / kTile) * kTile + q % kTile;
+    const int64_t p = perm[s];
+    px[o] = coords[2 * p];
+    py[o] = coords[2 * p + 1];
+    N4[o] = make_float4(shapeN[4 * p], shapeN[4 * p + 1], shapeN[4 * p + 2], shapeN[4 * p + 3]);
 }
 
 __global__ void k_tile_entries(const int4* __restrict__ tn, int64_t nt, int32_t* __restrict__ keys,
@@ -684,7 +1031,7 @@ struct d2ic_cg_plan {
     int device = 0;
     int64_t np = 0, nn = 0, ntiles = 0, nchunks = 0;
     int nb_maxdeg = 0;
-    int4* tile_hdr = nullptr;
+    int32_t* tile_cnt = nullptr;
     int4* tile_nodes = nullptr;
     float* px = nullptr;
     float* py = nullptr;
@@ -698,6 +1045,7 @@ struct d2ic_cg_plan {
     float* G[2] = {nullptr, nullptr};
     float* D[2] = {nullptr, nullptr};
     float* TP = nullptr;
+    float* TV = nullptr;
     double* NCP = nullptr;
     int ref_H = 0, ref_W = 0, interp = 0;
     bool has_ref = false;
@@ -806,7 +1154,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
     pl->np = n_pixels;
     pl->nn = n_nodes;
     pl->nb_maxdeg = nb_max_degree;
-    pl->nchunks = ceil_div(n_nodes, kWave);
+    pl->nchunks = ceil_div(n_nodes, kNodesPerItem);
     const int tb = 256;
 
     // validate indices once (a bad index would otherwise be read out of bounds)
@@ -843,10 +1191,6 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
     PA(pl->node_ptr, n_nodes + 1);
     PA(pl->nb_idx, n_nodes * nb_max_degree);
     PA(pl->nb_wp, n_nodes * nb_max_degree);
-    PA(pl->px, n_pixels);
-    PA(pl->py, n_pixels);
-    PA(pl->N4, n_pixels);
-    PA(pl->F, n_pixels);
     if (n_nodes * nb_max_degree > 0) {
         k_reg_table<<<ceil_div(n_nodes * nb_max_degree, tb), tb, 0, st>>>(nb_index, nb_degree, nb_weight, reg_weight,
                                                                           n_nodes, nb_max_degree, pl->nb_idx,
@@ -903,15 +1247,34 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PH(hipMemcpyAsync(&nt, tstart.as<int32_t>() + ng, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         PH(hipStreamSynchronize(st));
         pl->ntiles = nt;
-        PA(pl->tile_hdr, nt);
+        const int64_t npad = (int64_t)nt * kTile;
+        PA(pl->tile_cnt, nt);
         PA(pl->tile_nodes, nt);
         PA(pl->TP, 8 * (size_t)nt);
+        PA(pl->TV, (size_t)kTV * nt);
         PA(pl->node_ent, 4 * (size_t)nt);
+        PA(pl->px, npad);
+        PA(pl->py, npad);
+        PA(pl->N4, npad);
+        PA(pl->F, npad);
         k_tiles<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), tstart.as<int32_t>(), ng, n_pixels,
-                                                 perm.as<int32_t>(), pixel_nodes, pl->tile_hdr, pl->tile_nodes);
+                                                 perm.as<int32_t>(), pixel_nodes, pl->tile_cnt, pl->tile_nodes);
         PH(hipGetLastError());
-        k_soa<<<ceil_div(n_pixels, tb), tb, 0, st>>>(perm.as<int32_t>(), n_pixels, pixel_coords, pixel_shapeN, pl->px,
-                                                     pl->py, pl->N4);
+
+        // group id of every sorted pixel (1-based) = inclusive scan of the heads
+        Tmp gid, tmp4;
+        PH(gid.alloc(sizeof(int32_t) * n_pixels, st));
+        size_t b7 = 0;
+        PH(hipcub::DeviceScan::InclusiveSum(nullptr, b7, head.as<uint8_t>(), gid.as<int32_t>(), (int)n_pixels, st));
+        PH(tmp4.alloc(b7, st));
+        PH(hipcub::DeviceScan::InclusiveSum(tmp4.p, b7, head.as<uint8_t>(), gid.as<int32_t>(), (int)n_pixels, st));
+        k_pad_init<<<ceil_div(npad, tb), tb, 0, st>>>(npad, pl->px, pl->py, pl->N4);
+        PH(hipGetLastError());
+        k_soa<<<ceil_div(n_pixels, tb), tb, 0, st>>>(perm.as<int32_t>(), gid.as<int32_t>(), gstart.as<int32_t>(),
+                                                     tstart.as<int32_t>(), n_pixels, pixel_coords, pixel_shapeN,
+                                                     pl->px, pl->py, pl->N4);
+        PH(hipGetLastError());
+        k_tv_static<<<ceil_div(nt, tb), tb, 0, st>>>(pl->tile_cnt, nt, pl->px, pl->py, pl->TV);
         PH(hipGetLastError());
 
         // node -> (tile, local node) gather table
@@ -966,9 +1329,10 @@ int d2ic_cg_set_reference(d2ic_cg_plan* plan, const float* ref_image, int32_t H,
     D2IC_CHECK_ARG(ref_image != nullptr && H > 0 && W > 0, "reference image must be non-empty");
     D2IC_CHECK_ARG(interp == D2IC_INTERP_CUBIC || interp == D2IC_INTERP_LINEAR, "unsupported interpolation %d",
                    interp);
-    if (plan->np > 0) {
-        k_sample_ref<<<ceil_div(plan->np, 256), 256, 0, st>>>(ref_image, H, W, interp, plan->px, plan->py, plan->np,
-                                                              plan->F);
+    const int64_t npad = plan->ntiles * kTile;
+    if (npad > 0) {
+        k_sample_ref<<<ceil_div(npad, 256), 256, 0, st>>>(ref_image, H, W, interp, plan->px, plan->py, npad,
+                                                          plan->F);
         D2IC_LAUNCH_CHECK();
     }
     plan->ref_H = H;
@@ -1003,7 +1367,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         D2IC_CHECK_ARG(n_iters[p] != nullptr, "n_iters[%d] must not be NULL", p);
         D2IC_CHECK_ARG(max_iter[p] >= 0 && max_iter[p] <= (1 << 20), "max_iter[%d] out of range", p);
         ProbDesc& d = hd[p];
-        d.tile_hdr = pl->tile_hdr;
+        d.tile_cnt = pl->tile_cnt;
         d.tile_nodes = pl->tile_nodes;
         d.px = pl->px;
         d.py = pl->py;
@@ -1020,6 +1384,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         d.D[0] = pl->D[0];
         d.D[1] = pl->D[1];
         d.TP = pl->TP;
+        d.TV = pl->TV;
         d.NCP = pl->NCP;
         d.img = def_images[p];
         d.hist = hist ? hist[p] : nullptr;
@@ -1042,16 +1407,14 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     D2IC_CHECK_ARG(total_items < (int64_t)INT32_MAX, "batch too large");
 
     const size_t off_S = ((sizeof(ProbDesc) * P + 255) / 256) * 256;
-    const size_t off_A = off_S + ((sizeof(ProbState) * 3 * P + 255) / 256) * 256;
-    const size_t off_B = off_A + ((sizeof(Accum) * 3 + 255) / 256) * 256;
-    const size_t bytes = off_B + 256;
-    Tmp ws;
-    D2IC_HIP(ws.alloc(bytes, st));
-    char* base = ws.as<char>();
+    const size_t off_W = off_S + ((sizeof(ProbState) * P + 255) / 256) * 256;
+    const size_t bytes = off_W + ((sizeof(Workspace) + 255) / 256) * 256;
+    Tmp mem;
+    D2IC_HIP(mem.alloc(bytes, st));
+    char* base = mem.as<char>();
     ProbDesc* d_desc = reinterpret_cast<ProbDesc*>(base);
     ProbState* d_S = reinterpret_cast<ProbState*>(base + off_S);
-    Accum* d_acc = reinterpret_cast<Accum*>(base + off_A);
-    Barrier* d_bar = reinterpret_cast<Barrier*>(base + off_B);
+    Workspace* d_ws = reinterpret_cast<Workspace*>(base + off_W);
 
     Staging& sg = g_stage;
     if (sg.pending) {
@@ -1070,26 +1433,28 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     D2IC_HIP(hipMemcpyAsync(d_desc, sg.host, sizeof(ProbDesc) * P, hipMemcpyHostToDevice, st));
     D2IC_HIP(hipEventRecord(sg.ev, st));
     sg.pending = true;
-    D2IC_HIP(hipMemsetAsync(d_acc, 0, sizeof(Accum) * 3, st));
-    D2IC_HIP(hipMemsetAsync(d_bar, 0, sizeof(Barrier), st));
+    D2IC_HIP(hipMemsetAsync(d_ws, 0, sizeof(Workspace), st));
 
     int maxnn = 1;
-    for (int p = 0; p < P; ++p) maxnn = std::max<int>(maxnn, (int)std::max<int64_t>(plans[p]->nn * 2, 3 * max_iter[p]));
+    for (int p = 0; p < P; ++p)
+        maxnn = std::max<int>(maxnn, (int)std::max<int64_t>(std::max<int64_t>(plans[p]->nn * 2, 3 * max_iter[p]),
+                                                            4 * plans[p]->ntiles));
     dim3 gi((unsigned)std::min<int64_t>(ceil_div(maxnn, 256), 1024), (unsigned)P);
-    k_init<<<gi, 256, 0, st>>>(d_desc, P, d_S);
+    k_init<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_ws);
     D2IC_LAUNCH_CHECK();
 
     int coop_max = 0;
     int rc = occupancy_blocks(&coop_max);
     if (rc != D2IC_OK) return rc;
-    int64_t want = ceil_div(total_items, (int64_t)kWavesPerBlock * 2);
-    int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>(coop_max, want));
+    // enough blocks for ~4 tiles per wave, at least one XCD-round (8), at most
+    // what can be co-resident; a multiple of 8 keeps the XCD-aware split exact
+    int64_t want = ceil_div(total_items, (int64_t)kWavesPerBlock * 4);
+    want = std::max<int64_t>(8, ceil_div(want, 8) * 8);
+    int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kGroup * kMaxGroups), want);
     int max_steps = 8 + 2 * (2 + max_it * 12);
-    uint64_t timeout_ticks = 100000000ull * 4;  // wall_clock64 runs at 100 MHz: 4 s per barrier wait
-    void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_acc, (void*)&d_bar, (void*)&max_steps,
-                    (void*)&timeout_ticks};
+    void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps};
     D2IC_HIP(hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(nblocks), dim3(kBlock), args, 0, st));
-    k_finalize<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_bar);
+    k_finalize<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_ws);
     D2IC_LAUNCH_CHECK();
     return D2IC_OK;
 }

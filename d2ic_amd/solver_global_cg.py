@@ -97,14 +97,22 @@ class CGPlan:
         self._ref = ref
         return ref
 
-    def __del__(self):
+    def close(self):
+        """Free the plan's device memory now (also done on garbage collection)."""
         h = getattr(self, "handle", None)
         if h is not None and h.value:
-            try:
-                _lib.load().d2ic_cg_plan_destroy(h)
-            except Exception:
-                pass
+            _lib.load().d2ic_cg_plan_destroy(h)
             self.handle = ctypes.c_void_p(0)
+
+    def __del__(self):
+        import sys
+
+        if sys.is_finalizing():  # the HIP runtime may already be torn down
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GlobalCGSolver(SolverBase):
@@ -149,6 +157,13 @@ class GlobalCGSolver(SolverBase):
     @property
     def plan(self) -> CGPlan | None:
         return self._plan
+
+    def close(self) -> None:
+        """Release the device plan."""
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
+            self._compiled = False
 
     def solve(self, state: Any, def_image: Array) -> GlobalCGResult:
         if not self._compiled or self._plan is None:

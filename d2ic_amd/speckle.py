@@ -130,3 +130,45 @@ def rect_mask(H: int, W: int, side_h: int, side_w: int) -> np.ndarray:
     c0 = (W - side_w) // 2
     m[r0:r0 + side_h, c0:c0 + side_w] = True
     return m
+
+
+# ----------------------------------------------------------------------------
+# device renderer (large workloads)
+# ----------------------------------------------------------------------------
+
+def spots_array(spots: Spots) -> np.ndarray:
+    return np.ascontiguousarray(np.stack([spots.cx, spots.cy, spots.sigma, spots.amp], axis=1), dtype=np.float64)
+
+
+def render_spots_device(spots: Spots, H: int, W: int):
+    """Same sum as ``render_spots`` computed by the HIP renderer; float64 (H, W) device tensor."""
+    from . import _lib
+
+    t = _lib.require_gpu()
+    arr = t.as_tensor(spots_array(spots), device=_lib.device())
+    out = t.empty((H, W), dtype=t.float64, device=_lib.device())
+    _lib.call("d2ic_render_speckle", _lib.ptr(arr), int(arr.shape[0]), int(H), int(W), _lib.ptr(out), _lib.stream())
+    return out
+
+
+def speckle_sequence_device(H: int, W: int, seed: int, disp_fns, noise_sigma: float = 1.0):
+    """Reference image and one deformed image per displacement function, all
+    float32 device tensors (spots, scaling and noise as in ``speckle_pair``)."""
+    from . import _lib
+
+    t = _lib.require_gpu()
+    spots = speckle_spots(H, W, seed)
+    S_ref = render_spots_device(spots, H, W)
+    smin, smax = float(S_ref.min()), float(S_ref.max())
+    ref = (255.0 * (S_ref - smin) / (smax - smin)).to(t.float32)
+    frames = []
+    gen = t.Generator(device=_lib.device())
+    for k, fn in enumerate(disp_fns):
+        S = render_spots_device(displaced(spots, fn), H, W)
+        img = (255.0 * (S - smin) / (smax - smin)).to(t.float32)
+        if noise_sigma > 0:
+            gen.manual_seed(int(seed) * 7919 + 1000003 + k)
+            img = img + noise_sigma * t.randn(img.shape, generator=gen, device=_lib.device(), dtype=t.float32)
+        frames.append(img.contiguous())
+        del S
+    return ref.contiguous(), frames
