@@ -45,6 +45,18 @@ def build_workload(n_frames: int):
     return ref, frames, mask
 
 
+def measured_traffic():
+    """HBM bytes per k_cg_persistent launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or None."""
+    f = ROOT / "profiles" / "k_cg_persistent_plate2048_traffic.md"
+    if not f.exists():
+        return None
+    import re
+
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(ref_np, frames_np, mesh, mask, n_frames: int):
     """The oracle (a CPU restatement of the reference algorithm, FP32) on the host cores."""
     from oracle import oracle as O
@@ -71,8 +83,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-frames", type=int, default=1)
+    ap.add_argument("--cpu-frames", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path (--gpus must be 1).",
@@ -80,6 +94,16 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_calibration} MiB 5 times")
+        return 0
 
     from d2ic_amd import DICMeshBased, GlobalCGSolver, MeshDICConfig, mask_to_mesh_assets
 
@@ -108,7 +132,7 @@ def main() -> int:
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
-    kt, passes, iters = [], [], []
+    kt, kms, passes, iters = [], [], [], []
     u_prev = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -122,13 +146,16 @@ def main() -> int:
         u_prev = res.u_nodal
         kt.append((ev0, ev1))
         passes.append(solver.last_pass_count)
+        kms.append(solver.plan.kernel_ms())
         iters.append(res.diagnostics.info["n_iters"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     solve_ms = [a.elapsed_time(b) for a, b in kt]
     fps = K / elapsed
     tot_bytes = B_pass * float(sum(passes))
-    achieved = tot_bytes / (sum(solve_ms) * 1e-3) / 1e9
+    # algorithmic bytes of the persistent CG kernel / its measured launch time
+    achieved = tot_bytes / (sum(kms) * 1e-3) / 1e9
+    traffic = measured_traffic()
     log(f"[bench] {K} frames in {elapsed:.3f} s -> {fps:.2f} frames/s; iters {iters}; passes {passes}")
 
     cpu = None
@@ -164,9 +191,11 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_cg_persistent (+ init/finalize/strain kernels inside the timed events)",
+            "traffic": traffic,
+            "kernel": "k_cg_persistent (one launch per frame)",
+            "algorithmic_bytes_per_launch": B_pass * float(np.mean(passes)),
             "bytes_per_pass": B_pass,
+            "mean_kernel_ms": float(np.mean(kms)),
             "mean_solve_ms": float(np.mean(solve_ms)),
         },
         "cpu_baseline": cpu,

@@ -36,6 +36,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -45,7 +46,8 @@
 
 namespace d2ic {
 
-constexpr int kTile = 64;
+constexpr int kTile = 256;  // pixels per tile (one element, or a 256-pixel run of one)
+constexpr int kChunks = kTile / 64;  // a wave processes a tile as kChunks chunks of 64 pixels
 constexpr int kBlock = 1024;  // 16 waves: one block per CU
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxProblems = 256;
@@ -177,8 +179,8 @@ constexpr int kNodesPerWave = kWave / kLanesPerNode;
 constexpr int kNodesPerItem = kNodesPerWave * kWavesPerBlock;  // node item = one block
 constexpr int kBoxMax = 512;  // floats of staged image per wave (8 loads per lane)
 
-struct TilePix {  // per-lane pixel data of one tile
-    float x, y, f;
+struct TilePix {  // per-lane pixel data of one chunk (f is loaded at use)
+    float x, y;
     float4 N;
 };
 
@@ -186,14 +188,13 @@ __device__ __forceinline__ float tile_tv(const ProbDesc* __restrict__ D, int t, 
     return lane < 29 ? gld(D->TV)[(size_t)t * kTV + lane] : 0.f;
 }
 
-__device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int t, int lane) {
+__device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int t, int c, int lane) {
     TilePix r;
-    const size_t i = (size_t)t * kTile + lane;
+    const size_t i = (size_t)t * kTile + c * kWave + lane;
     r.x = gld(D->px)[i];
     r.y = gld(D->py)[i];
     const f4v nv = gld(reinterpret_cast<const f4v*>(D->N4))[i];
     r.N = make_float4(nv.x, nv.y, nv.z, nv.w);
-    r.f = gld(D->F)[i];
     return r;
 }
 
@@ -201,48 +202,65 @@ __device__ __forceinline__ float bcast(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// Candidate node displacements of a tile (wave-uniform) and the box of image
-// rows/columns its samples can touch: static pixel bbox widened by the node
-// displacement range (+1 px margin each side) and the cubic support.
-struct TileGeo {
-    float2 uc[4];
-    int cnt, bx0, by0, bw, bh;
-    bool staged;
-};
+__device__ __forceinline__ float uni(float v) {  // wave-uniform value -> SGPR
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 
-__device__ __forceinline__ TileGeo tile_geo(float tv, int flags, float alpha, float beta) {
+// Candidate node displacements of a tile (wave-uniform, from the tile record).
+__device__ __forceinline__ void tile_uc(float tv, int flags, float alpha, float beta, float2 (&uc)[4]) {
     const bool restart = (flags & F_RESTART) != 0;
-    TileGeo g;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const float2 u = make_float2(bcast(tv, 6 * a), bcast(tv, 6 * a + 1));
         const float2 gr = make_float2(bcast(tv, 6 * a + 2), bcast(tv, 6 * a + 3));
         const float2 dp = make_float2(bcast(tv, 6 * a + 4), bcast(tv, 6 * a + 5));
-        g.uc[a] = step_to(u, direction(gr, dp, beta, restart), alpha);
+        const float2 c = step_to(u, direction(gr, dp, beta, restart), alpha);
+        uc[a] = make_float2(uni(c.x), uni(c.y));
     }
+}
+
+// Pixel count and the box of image rows/columns a tile's samples can touch:
+// static pixel bbox widened by the node displacement range and the cubic
+// support.  All fields wave-uniform.
+struct TileBox {
+    int cnt, bx0, by0, bw, bh;
+    bool staged;
+};
+
+__device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
+    TileBox g;
     g.cnt = __float_as_int(bcast(tv, 24));
-    const float uxmin = fminf(fminf(g.uc[0].x, g.uc[1].x), fminf(g.uc[2].x, g.uc[3].x));
-    const float uxmax = fmaxf(fmaxf(g.uc[0].x, g.uc[1].x), fmaxf(g.uc[2].x, g.uc[3].x));
-    const float uymin = fminf(fminf(g.uc[0].y, g.uc[1].y), fminf(g.uc[2].y, g.uc[3].y));
-    const float uymax = fmaxf(fmaxf(g.uc[0].y, g.uc[1].y), fmaxf(g.uc[2].y, g.uc[3].y));
+    const float uxmin = fminf(fminf(uc[0].x, uc[1].x), fminf(uc[2].x, uc[3].x));
+    const float uxmax = fmaxf(fmaxf(uc[0].x, uc[1].x), fmaxf(uc[2].x, uc[3].x));
+    const float uymin = fminf(fminf(uc[0].y, uc[1].y), fminf(uc[2].y, uc[3].y));
+    const float uymax = fmaxf(fmaxf(uc[0].y, uc[1].y), fmaxf(uc[2].y, uc[3].y));
     g.staged = (uxmax - uxmin) < 64.f && (uymax - uymin) < 64.f && fabsf(uxmin) < 1e6f && fabsf(uymin) < 1e6f;
     g.bx0 = g.by0 = g.bw = g.bh = 0;
     if (g.staged) {
-        g.bx0 = __float_as_int(bcast(tv, 25)) + (int)floorf(uxmin) - 2;
+        // cell index of a sample = pixel column + floor(u); cubic taps span
+        // cell - 1 .. cell + 2; one more column/row each side absorbs rounding
+        // of the interpolated u just outside [umin, umax]
+        g.bx0 = __float_as_int(bcast(tv, 25)) + (int)floorf(uxmin) - 1;
         const int bx1 = __float_as_int(bcast(tv, 26)) + (int)floorf(uxmax) + 3;
-        g.by0 = __float_as_int(bcast(tv, 27)) + (int)floorf(uymin) - 2;
+        g.by0 = __float_as_int(bcast(tv, 27)) + (int)floorf(uymin) - 1;
         const int by1 = __float_as_int(bcast(tv, 28)) + (int)floorf(uymax) + 3;
         g.bw = bx1 - g.bx0 + 1;
         g.bh = by1 - g.by0 + 1;
         g.staged = g.bw * g.bh <= kBoxMax;
     }
+    g.cnt = __builtin_amdgcn_readfirstlane(g.cnt);
+    g.bx0 = __builtin_amdgcn_readfirstlane(g.bx0);
+    g.by0 = __builtin_amdgcn_readfirstlane(g.by0);
+    g.bw = __builtin_amdgcn_readfirstlane(g.bw);
+    g.bh = __builtin_amdgcn_readfirstlane(g.bh);
+    g.staged = __builtin_amdgcn_readfirstlane((int)g.staged) != 0;
     return g;
 }
 
 constexpr int kBoxRegs = kBoxMax / kWave;
 
 // Loads of the staged box into registers (issued one tile ahead of use).
-__device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const TileGeo& g, int lane,
+__device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const TileBox& g, int lane,
                                          float (&v)[kBoxRegs]) {
     if (!g.staged) return;
     const int n = g.bw * g.bh, interp = D->interp, H = D->H, W = D->W;
@@ -260,7 +278,7 @@ __device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const T
     }
 }
 
-__device__ __forceinline__ void box_store(const TileGeo& g, int lane, const float (&v)[kBoxRegs],
+__device__ __forceinline__ void box_store(const TileBox& g, int lane, const float (&v)[kBoxRegs],
                                           __attribute__((address_space(3))) float* box) {
     if (!g.staged) return;
     const int n = g.bw * g.bh;
@@ -273,15 +291,18 @@ __device__ __forceinline__ void box_store(const TileGeo& g, int lane, const floa
     __builtin_amdgcn_wave_barrier();
 }
 
-// A-step on one tile: residual at the candidate, its sum of squares (lane 63)
-// and the tile's 8 gradient partials.  Samples inside the staged box read LDS;
-// others read global memory (same values).
-__device__ __forceinline__ double tile_compute(const ProbDesc* __restrict__ D, int t, const TilePix& px,
-                                               const TileGeo& g, int lane,
-                                               const __attribute__((address_space(3))) float* box) {
+// A-step on one 64-pixel chunk of a tile: residual at the candidate,
+// accumulated per lane into r^2 (FP64) and the 8 gradient partials
+// acc[2a + {0,1}] += r * (dg/dx, dg/dy) * N_a.  Samples inside the staged box
+// read LDS; others read global memory (same values).  Dead lanes add zeros.
+__device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, size_t i, const TilePix& px,
+                                            const float2 (&uc)[4], const TileBox& g, bool live,
+                                            const __attribute__((address_space(3))) float* box, float (&acc)[8],
+                                            double& r2) {
+    const float f = gld(D->F)[i];
     const float4 N = px.N;
-    const float ux = N.x * g.uc[0].x + N.y * g.uc[1].x + N.z * g.uc[2].x + N.w * g.uc[3].x;
-    const float uy = N.x * g.uc[0].y + N.y * g.uc[1].y + N.z * g.uc[2].y + N.w * g.uc[3].y;
+    const float ux = N.x * uc[0].x + N.y * uc[1].x + N.z * uc[2].x + N.w * uc[3].x;
+    const float uy = N.x * uc[0].y + N.y * uc[1].y + N.z * uc[2].y + N.w * uc[3].y;
     int x0, y0;
     float fx, fy;
     split_coord(px.x, ux, x0, fx);
@@ -293,22 +314,50 @@ __device__ __forceinline__ double tile_compute(const ProbDesc* __restrict__ D, i
         sample_box(D->interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
     else
         sample_at(D->interp, D->img, D->H, D->W, x0, fx, y0, fy, gv, gx, gy);
-    const float r = lane < g.cnt ? gv - px.f : 0.f;
-    const double r2 = wave_sum63((double)r * (double)r);
+    const float r = live ? gv - f : 0.f;
+    r2 += (double)r * (double)r;
     const float qx = r * gx, qy = r * gy;
-    const float s0 = wave_sum63(qx * N.x), s1 = wave_sum63(qy * N.x);
-    const float s2 = wave_sum63(qx * N.y), s3 = wave_sum63(qy * N.y);
-    const float s4 = wave_sum63(qx * N.z), s5 = wave_sum63(qy * N.z);
-    const float s6 = wave_sum63(qx * N.w), s7 = wave_sum63(qy * N.w);
-    if (lane == kWave - 1) {
-        D2IC_GLOBAL f4v* o = gst(reinterpret_cast<f4v*>(D->TP + 8 * (size_t)t));
-        f4v a0, a1;
-        a0.x = s0; a0.y = s1; a0.z = s2; a0.w = s3;
-        a1.x = s4; a1.y = s5; a1.z = s6; a1.w = s7;
-        o[0] = a0;
-        o[1] = a1;
+    acc[0] += qx * N.x;
+    acc[1] += qy * N.x;
+    acc[2] += qx * N.y;
+    acc[3] += qy * N.y;
+    acc[4] += qx * N.z;
+    acc[5] += qy * N.z;
+    acc[6] += qx * N.w;
+    acc[7] += qy * N.w;
+}
+
+// Wave sums of 8 per-lane values in one transposed pass: three exchange
+// steps inside each 8-lane group (row_half_mirror, quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1]) leave lane i with the group sum of value
+// 4*bit2(i) + 2*bit0(i) + bit1(i); row_ror:8 and two xor shuffles (16, 32)
+// add the 8 groups.  Lanes 0..7 store the 8 totals.  Fixed association order.
+__device__ __forceinline__ void reduce8_store(const float (&v)[8], int lane, D2IC_GLOBAL float* out) {
+    const bool b2 = (lane & 4) != 0, b0 = (lane & 1) != 0, b1 = (lane & 2) != 0;
+    float w[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const float keep = b2 ? v[p + 4] : v[p];
+        const float send = b2 ? v[p] : v[p + 4];
+        w[p] = keep + dpp_f<0x141>(send);
     }
-    return r2;
+    float x[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const float keep = b0 ? w[q + 2] : w[q];
+        const float send = b0 ? w[q] : w[q + 2];
+        x[q] = keep + dpp_f<0xB1>(send);
+    }
+    float y;
+    {
+        const float keep = b1 ? x[1] : x[0];
+        const float send = b1 ? x[0] : x[1];
+        y = keep + dpp_f<0x4E>(send);
+    }
+    y += dpp_f<0x128>(y);
+    y += __shfl_xor(y, 16, kWave);
+    y += __shfl_xor(y, 32, kWave);
+    if (lane < 8) out[(b2 ? 4 : 0) + (b0 ? 2 : 0) + (b1 ? 1 : 0)] = y;
 }
 
 // sum over the 8 lanes of an aligned lane group (DPP quad_perm and
@@ -660,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
         }
 
         // ---- tiles (A only): contiguous share per wave, software-pipelined:
-        //      record 2 tiles ahead, pixel data and staged box 1 tile ahead ----
+        //      record 2 tiles ahead, staged box 1 tile ahead, pixels 1 chunk ahead ----
         if (isA && nT > 0) {
             const int t0 = (int)(((long long)nT * gwave) / nwaves);
             const int t1 = (int)(((long long)nT * (gwave + 1)) / nwaves);
@@ -674,12 +723,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                 pa = next_prob(pa, t0);
                 unsigned long long hi = 0ull, lo = 0ull;
                 int bad = 0;
-                // prologue: tile t0 record, geometry, box and pixels; t0+1 record
-                float tv_cur = tile_tv(descs + pa, t0 - s_toff[pa], lane);
-                TileGeo g_cur = tile_geo(tv_cur, s_flags[pa], s_alpha[pa], s_beta[pa]);
+                // prologue: tile t0 record, geometry, box and first chunk; t0+1 record
+                float2 uc[4];
+                const float tv_cur = tile_tv(descs + pa, t0 - s_toff[pa], lane);
+                tile_uc(tv_cur, s_flags[pa], s_alpha[pa], s_beta[pa], uc);
+                TileBox b_cur = tile_box(tv_cur, uc);
                 float R[kBoxRegs];
-                box_load(descs + pa, g_cur, lane, R);
-                TilePix px_cur = tile_pix(descs + pa, t0 - s_toff[pa], lane);
+                box_load(descs + pa, b_cur, lane, R);
+                TilePix px = tile_pix(descs + pa, t0 - s_toff[pa], 0, lane);
                 int pb = pa;
                 float tv_nxt = 0.f;
                 if (t0 + 1 < t1) {
@@ -687,24 +738,38 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                     tv_nxt = tile_tv(descs + pb, t0 + 1 - s_toff[pb], lane);
                 }
                 for (int it = t0; it < t1; ++it) {
-                    // stage 1: box of this tile into LDS
-                    box_store(g_cur, lane, R, my_box);
-                    // stage 2: next tile's geometry and box loads; record two ahead; next pixels
-                    TileGeo g_nxt;
-                    TilePix px_nxt;
+                    const ProbDesc* Da = descs + pa;
+                    const int ta = it - s_toff[pa];
+                    // this tile's box into LDS, then the next tile's geometry and box loads
+                    box_store(b_cur, lane, R, my_box);
+                    TileBox b_nxt;
                     float tv_nn = 0.f;
                     int pc = pb;
                     if (it + 1 < t1) {
-                        g_nxt = tile_geo(tv_nxt, s_flags[pb], s_alpha[pb], s_beta[pb]);
-                        box_load(descs + pb, g_nxt, lane, R);
+                        float2 ucn[4];
+                        tile_uc(tv_nxt, s_flags[pb], s_alpha[pb], s_beta[pb], ucn);
+                        b_nxt = tile_box(tv_nxt, ucn);
+                        box_load(descs + pb, b_nxt, lane, R);
                         if (it + 2 < t1) {
                             pc = next_prob(pb, it + 2);
                             tv_nn = tile_tv(descs + pc, it + 2 - s_toff[pc], lane);
                         }
-                        px_nxt = tile_pix(descs + pb, it + 1 - s_toff[pb], lane);
                     }
-                    // stage 3: this tile
-                    const double r2 = tile_compute(descs + pa, it - s_toff[pa], px_cur, g_cur, lane, my_box);
+                    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    double r2 = 0.0;
+                    const int nch = (b_cur.cnt + kWave - 1) / kWave;
+                    for (int c = 0; c < nch; ++c) {
+                        TilePix pn;
+                        if (c + 1 < nch)
+                            pn = tile_pix(Da, ta, c + 1, lane);
+                        else if (it + 1 < t1)
+                            pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
+                        chunk_accum(Da, (size_t)ta * kTile + c * kWave + lane, px, uc, b_cur,
+                                    c * kWave + lane < b_cur.cnt, my_box, acc, r2);
+                        px = pn;
+                    }
+                    reduce8_store(acc, lane, gst(Da->TP + 8 * (size_t)ta));
+                    r2 = wave_sum63(r2);
                     if (lane == kWave - 1) {
                         if (!(r2 <= 1e300))
                             bad = 1;
@@ -720,10 +785,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                         hi = lo = 0ull;
                         bad = 0;
                     }
+                    if (it + 1 < t1) tile_uc(tv_nxt, s_flags[pb], s_alpha[pb], s_beta[pb], uc);
                     pa = pb;
                     pb = pc;
-                    g_cur = g_nxt;
-                    px_cur = px_nxt;
+                    b_cur = b_nxt;
                     tv_nxt = tv_nn;
                 }
             }
@@ -791,8 +856,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
     }
 }
 
+// Zero a word range (used instead of hipMemsetAsync so that the launch
+// sequence of a solve can be captured into a HIP graph).
+__global__ void k_zero_words(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 // Per-problem start state, first plan (an A-step at alpha = 0, i.e. the value
 // and gradient at u0), zeroed g/d buffers, NaN-filled history.
+
 __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S,
                        Workspace* __restrict__ ws) {
     const int p = blockIdx.y;
@@ -1051,6 +1124,9 @@ struct d2ic_cg_plan {
     bool has_ref = false;
     size_t bytes = 0;
     std::vector<void*> allocs;
+    // start/end of the last persistent-kernel launch this plan led (plans[0])
+    hipEvent_t ev_start = nullptr, ev_end = nullptr;
+    bool timed = false;
 };
 
 namespace {
@@ -1307,6 +1383,8 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
 
 int d2ic_cg_plan_destroy(d2ic_cg_plan* plan) {
     if (!plan) return D2IC_OK;
+    if (plan->ev_start) (void)hipEventDestroy(plan->ev_start);
+    if (plan->ev_end) (void)hipEventDestroy(plan->ev_end);
     for (void* p : plan->allocs) (void)hipFree(p);
     delete plan;
     return D2IC_OK;
@@ -1319,6 +1397,17 @@ int d2ic_cg_plan_info(const d2ic_cg_plan* plan, int64_t* n_pixels, int64_t* n_no
     if (n_nodes) *n_nodes = plan->nn;
     if (n_tiles) *n_tiles = plan->ntiles;
     if (device_bytes) *device_bytes = (int64_t)plan->bytes;
+    return D2IC_OK;
+}
+
+int d2ic_cg_kernel_ms(const d2ic_cg_plan* plan, float* ms) {
+    D2IC_CHECK_ARG(plan != nullptr && ms != nullptr, "plan and ms must not be NULL");
+    if (!plan->timed) {
+        set_error("no solve has been launched with this plan first in the batch");
+        return D2IC_ERR_STATE;
+    }
+    D2IC_HIP(hipEventSynchronize(plan->ev_end));
+    D2IC_HIP(hipEventElapsedTime(ms, plan->ev_start, plan->ev_end));
     return D2IC_OK;
 }
 
@@ -1353,7 +1442,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
                    "batch arrays must not be NULL");
     std::vector<ProbDesc> hd(P);
     int max_it = 0;
-    int64_t total_items = 0;
+    int64_t total_items = 0, total_work = 0;
     for (int p = 0; p < P; ++p) {
         d2ic_cg_plan* pl = plans[p];
         D2IC_CHECK_ARG(pl != nullptr, "plan %d is NULL", p);
@@ -1403,6 +1492,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         d.invNp = 1.0 / (double)(pl->np > 0 ? pl->np : 1);
         max_it = std::max(max_it, max_iter[p]);
         total_items += pl->ntiles + pl->nchunks;
+        total_work += pl->ntiles * kChunks + pl->nchunks;
     }
     D2IC_CHECK_ARG(total_items < (int64_t)INT32_MAX, "batch too large");
 
@@ -1433,7 +1523,9 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     D2IC_HIP(hipMemcpyAsync(d_desc, sg.host, sizeof(ProbDesc) * P, hipMemcpyHostToDevice, st));
     D2IC_HIP(hipEventRecord(sg.ev, st));
     sg.pending = true;
-    D2IC_HIP(hipMemsetAsync(d_ws, 0, sizeof(Workspace), st));
+    static_assert(sizeof(Workspace) % 4 == 0, "workspace is cleared in 32-bit words");
+    k_zero_words<<<64, 256, 0, st>>>(reinterpret_cast<uint32_t*>(d_ws), (int64_t)(sizeof(Workspace) / 4));
+    D2IC_LAUNCH_CHECK();
 
     int maxnn = 1;
     for (int p = 0; p < P; ++p)
@@ -1446,14 +1538,34 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     int coop_max = 0;
     int rc = occupancy_blocks(&coop_max);
     if (rc != D2IC_OK) return rc;
-    // enough blocks for ~4 tiles per wave, at least one XCD-round (8), at most
-    // what can be co-resident; a multiple of 8 keeps the XCD-aware split exact
-    int64_t want = ceil_div(total_items, (int64_t)kWavesPerBlock * 4);
+    // enough blocks for ~4 chunks of 64 pixels per wave, at least one XCD-round
+    // (8), at most what can be co-resident; a multiple of 8 keeps the XCD-aware
+    // split exact
+    int64_t want = ceil_div(total_work, (int64_t)kWavesPerBlock * 4);
     want = std::max<int64_t>(8, ceil_div(want, 8) * 8);
     int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kGroup * kMaxGroups), want);
     int max_steps = 8 + 2 * (2 + max_it * 12);
     void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps};
-    D2IC_HIP(hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(nblocks), dim3(kBlock), args, 0, st));
+    d2ic_cg_plan* lead = plans[0];
+    if (!lead->ev_start) D2IC_HIP(hipEventCreate(&lead->ev_start));
+    if (!lead->ev_end) D2IC_HIP(hipEventCreate(&lead->ev_end));
+    D2IC_HIP(hipEventRecord(lead->ev_start, st));
+    // D2IC_PLAIN_LAUNCH=1 launches the same grid with an ordinary launch (for
+    // tools that cannot follow cooperative launches).  The grid never exceeds
+    // the co-resident capacity; if blocks were not all resident the bounded
+    // barrier wait would end the solve with n_iters = -1 instead of hanging.
+    static const bool plain_launch = [] {
+        const char* e = getenv("D2IC_PLAIN_LAUNCH");
+        return e && e[0] == '1';
+    }();
+    if (plain_launch) {
+        k_cg_persistent<<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps);
+        D2IC_LAUNCH_CHECK();
+    } else {
+        D2IC_HIP(hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(nblocks), dim3(kBlock), args, 0, st));
+    }
+    D2IC_HIP(hipEventRecord(lead->ev_end, st));
+    lead->timed = true;
     k_finalize<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_ws);
     D2IC_LAUNCH_CHECK();
     return D2IC_OK;

@@ -84,6 +84,12 @@ class CGPlan:
                   ctypes.byref(d))
         return dict(n_pixels=a.value, n_nodes=b.value, n_tiles=c.value, device_bytes=d.value)
 
+    def kernel_ms(self) -> float:
+        """Duration of the last persistent CG kernel launch this plan led (waits for it)."""
+        ms = ctypes.c_float(0.0)
+        _lib.call("d2ic_cg_kernel_ms", self.handle, ctypes.byref(ms))
+        return float(ms.value)
+
     def set_reference(self, ref_image, interpolation: str):
         t = _lib.torch()
         ref = _lib.to_dev(ref_image, t.float32)

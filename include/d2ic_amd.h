@@ -112,6 +112,12 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
                         const int32_t* max_iter, const float* tol, float* const* hist, int32_t* const* n_iters,
                         int32_t* const* pass_count, void* stream);
 
+/* Duration in ms of the last persistent CG kernel launch that this plan led
+ * (first plan of the batch), from HIP events recorded around that launch on
+ * its stream; waits for the launch to finish.  No reference counterpart
+ * (measurement hook). */
+int d2ic_cg_kernel_ms(const d2ic_cg_plan* plan, float* ms);
+
 /* ---------------------------------------------------------------------------
  * Translation ZNCC (replaces _match_centers_jax, solver_translation_zncc.py:118-168)
  * ------------------------------------------------------------------------ */
