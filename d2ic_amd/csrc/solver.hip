@@ -52,8 +52,6 @@ constexpr int kBlock = 1024;  // 16 waves: one block per CU
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxProblems = 256;
 constexpr int kSlots = 16;    // spread of the fixed-point residual sums
-constexpr int kGroup = 16;    // barrier fan-in
-constexpr int kMaxGroups = 64;
 constexpr double kFixScale = 1099511627776.0;  // 2^40
 
 enum Mode : int { MODE_INIT = 0, MODE_PASS = 1, MODE_NODE = 2, MODE_DONE = 3 };
@@ -109,17 +107,22 @@ struct PlanEntry {
     int pad[3];
 };
 
+constexpr int kMaxBlocks = 1024;
+constexpr int kArriveStride = 32;  // one 128-byte line per block's arrival flag
+
+// Per-launch workspace.  The residual accumulators are triple-buffered by
+// step (written during step s, read right after barrier s, cleared during
+// step s + 2 by block 0).
 struct Workspace {
-    unsigned long long hi[kMaxProblems][kSlots];
-    unsigned long long lo[kMaxProblems][kSlots];
-    int bad[kMaxProblems];
-    PlanEntry plan[kMaxProblems];
+    unsigned long long hi[3][kMaxProblems][kSlots];
+    unsigned long long lo[3][kMaxProblems][kSlots];
+    int bad[3][kMaxProblems];
+    PlanEntry plan[kMaxProblems];  // first step's plan (written by k_init)
     int n_tiles_active, n_chunks_active, any_active, pad0;
-    unsigned group_cnt[kMaxGroups];
-    unsigned root_cnt;
-    unsigned gen;
+    unsigned arrive[kMaxBlocks * kArriveStride];   // last step each block finished
+    unsigned release[kMaxBlocks * kArriveStride];  // last step every block finished (one line per block)
     int err;
-    int pad1;
+    int pad1[2];
 };
 
 typedef float f2v __attribute__((ext_vector_type(2)));
@@ -400,7 +403,7 @@ __device__ __forceinline__ void block_store2(double a, double b, double* s_red, 
 // A-step node item (128 nodes, 8 lanes per node): materialise d, g.d and the
 // spring energy at the candidate (both halves of every ordered pair).
 __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
-                            double* s_red) {
+                            double* s_red, int par) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
     const bool restart = (flags & F_RESTART) != 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -428,12 +431,13 @@ __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags,
             E += (double)(w * (dx * dx + dy * dy));
         }
     }
-    block_store2(gd, E, s_red, D->NCP + 2 * (size_t)item);
+    block_store2(gd, E, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
 }
 
 // B-step node item: new point, gradient gather, |g|^2, g.(g - g_prev), and
 // the scatter of (u, g, d) into the tile records for the next A-step.
-__device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, double* s_red) {
+__device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, double* s_red,
+                            int par) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int sub = lane % kLanesPerNode;
@@ -488,14 +492,14 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             st2(rec, 2, dn);
         }
     }
-    block_store2(gg, gnum, s_red, D->NCP + 2 * (size_t)item);
+    block_store2(gg, gnum, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
 }
 
 // State transition of one problem after a step (root block only).  just_did_a:
 // the finished step was an A-step (Armijo decision follows), else a B-step
 // (stop test and beta follow).  fin: node-chunk sums of the finished step.
 __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double sr2, bool bad, double fin0,
-                      double fin1) {
+                      double fin1, bool writer) {
     int flags = 0;
     if (st.mode == MODE_DONE) return 0;
     if (just_did_a) {
@@ -520,7 +524,7 @@ __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double 
             if (accept) {
                 flags |= F_DOB;
                 if (st.mode != MODE_INIT) {
-                    if (D.hist) {
+                    if (D.hist && writer) {
                         D.hist[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
                         D.hist[3 * st.k + 1] = st.gnorm;
                         D.hist[3 * st.k + 2] = st.alpha;
@@ -541,15 +545,15 @@ __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double 
             if (st.k >= D.max_iter) {
                 // while_loop exits on k < max_iter before evaluating anything
                 st.mode = MODE_DONE;
-                *D.n_iters = st.k;
+                if (writer) *D.n_iters = st.k;
             } else if (gnorm < D.tol) {
                 st.mode = MODE_DONE;
-                if (D.hist) {
+                if (D.hist && writer) {
                     D.hist[3 * st.k] = (float)st.J;
                     D.hist[3 * st.k + 1] = gnorm;
                     D.hist[3 * st.k + 2] = 0.f;
                 }
-                *D.n_iters = st.k;
+                if (writer) *D.n_iters = st.k;
             } else {
                 float b = 0.f;
                 if (st.k > 0) {
@@ -577,65 +581,87 @@ __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double 
     return flags;
 }
 
-// Root block: reduce the finished step, decide every problem, publish the plan.
-__device__ void root_step(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S, Workspace* ws,
-                          bool just_did_a, const int* s_flags, double (*s_fin)[2], double* s_sr2) {
+// Every block, right after barrier `step`: reduce the finished step from the
+// same memory in the same order, run the same state machine on its own copy
+// of the problem states, and derive the next step's plan (identical in every
+// block).  Block 0 alone writes histories, n_iters and the global states, and
+// clears the accumulators of step + 2.
+__device__ void block_decide(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S, Workspace* ws,
+                             int step, ProbState* s_st, int* s_flags, float* s_alpha, float* s_beta, int* s_toff,
+                             int* s_coff, int* s_hdr, double (*s_fin)[2], double* s_sr2, int* s_badp) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    // node-chunk sums in fixed order, one wave per problem
+    const bool just_did_a = (step & 1) == 0;
+    const int par = step & 1, buf = step % 3;
+    const bool writer = blockIdx.x == 0;
+    // node-chunk sums (fixed order, one wave per problem) and the exact
+    // fixed-point residual sums
     for (int p = wave; p < P; p += kWavesPerBlock) {
         const int fl = s_flags[p];
         const bool had = just_did_a ? (fl & F_DOA) : (fl & F_DOB);
         double a = 0.0, b = 0.0;
+        unsigned long long hi = 0ull, lo = 0ull;
         if (had) {
             const ProbDesc& D = descs[p];
+            const double* ncp = D.NCP + 2 * (size_t)par * D.nchunks;
             for (int c = lane; c < D.nchunks; c += kWave) {
-                a += __hip_atomic_load(&D.NCP[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                b += __hip_atomic_load(&D.NCP[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                a += __hip_atomic_load(&ncp[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                b += __hip_atomic_load(&ncp[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+        }
+        if (just_did_a && lane < kSlots) {
+            hi = __hip_atomic_load(&ws->hi[buf][p][lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lo = __hip_atomic_load(&ws->lo[buf][p][lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         a = wave_sum(a);
         b = wave_sum(b);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            hi += __shfl_xor(hi, off, kWave);
+            lo += __shfl_xor(lo, off, kWave);
+        }
         if (lane == 0) {
             s_fin[p][0] = a;
             s_fin[p][1] = b;
+            s_sr2[p] = (double)hi + (double)lo / kFixScale;
+            s_badp[p] = just_did_a ? __hip_atomic_load(&ws->bad[buf][p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : 0;
         }
-    }
-    for (int p = threadIdx.x; p < P; p += kBlock) {
-        unsigned long long hi = 0, lo = 0;
-        for (int k = 0; k < kSlots; ++k) {
-            hi += __hip_atomic_load(&ws->hi[p][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            lo += __hip_atomic_load(&ws->lo[p][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ws->hi[p][k] = 0ull;
-            ws->lo[p][k] = 0ull;
-        }
-        s_sr2[p] = (double)hi + (double)lo / kFixScale;
     }
     __syncthreads();
     for (int p = threadIdx.x; p < P; p += kBlock) {
-        ProbState st = S[p];
-        const int bad = ws->bad[p];
-        ws->bad[p] = 0;
-        const int fl = decide(descs[p], st, just_did_a, s_sr2[p], bad != 0, s_fin[p][0], s_fin[p][1]);
-        S[p] = st;
-        ws->plan[p].flags = fl;
-        ws->plan[p].alpha = st.alpha;
-        ws->plan[p].beta = st.beta;
+        ProbState st = s_st[p];
+        const int fl = decide(descs[p], st, just_did_a, s_sr2[p], s_badp[p] != 0, s_fin[p][0], s_fin[p][1], writer);
+        s_st[p] = st;
+        if (writer) S[p] = st;
+        s_flags[p] = fl;
+        s_alpha[p] = st.alpha;
+        s_beta[p] = st.beta;
+    }
+    if (writer) {
+        const int zb = (step + 2) % 3;
+        for (int i = threadIdx.x; i < P * kSlots; i += kBlock) {
+            __hip_atomic_store(&ws->hi[zb][0][0] + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ws->lo[zb][0][0] + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (int p = threadIdx.x; p < P; p += kBlock)
+            __hip_atomic_store(&ws->bad[zb][p], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int to = 0, co = 0, any = 0;
         for (int p = 0; p < P; ++p) {
-            const int fl = ws->plan[p].flags;
-            ws->plan[p].tile_off = to;
-            ws->plan[p].chunk_off = co;
+            const int fl = s_flags[p];
+            s_toff[p] = to;
+            s_coff[p] = co;
             if (fl & F_DOA) to += descs[p].ntiles;
             if (fl & (F_DOA | F_DOB)) co += descs[p].nchunks;
-            if (S[p].mode != MODE_DONE) any = 1;  // a problem whose trial was rejected idles one step
+            if (s_st[p].mode != MODE_DONE) any = 1;  // a problem whose trial was rejected idles one step
         }
-        ws->n_tiles_active = to;
-        ws->n_chunks_active = co;
-        ws->any_active = any;
+        s_hdr[0] = to;
+        s_hdr[1] = co;
+        s_hdr[2] = any;
     }
+    __syncthreads();
 }
 
 // wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
@@ -656,8 +682,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
     __shared__ int s_bad[kMaxProblems];
     __shared__ double s_fin[kMaxProblems][2];
     __shared__ double s_sr2[kMaxProblems];
+    __shared__ int s_badp[kMaxProblems];
+    __shared__ ProbState s_st[kMaxProblems];
     __shared__ int s_hdr[3];
-    __shared__ int s_root, s_ok;
+    __shared__ int s_ok;
     __shared__ double s_red[2 * kWavesPerBlock];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
     __attribute__((address_space(3))) float* my_box = (__attribute__((address_space(3))) float*)s_box[threadIdx.x / kWave];
@@ -670,32 +698,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
     const int vb = (nb % 8 == 0) ? (int)((blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8) : (int)blockIdx.x;
     const int gwave = __builtin_amdgcn_readfirstlane(vb * kWavesPerBlock + wave);
     const int nwaves = nb * kWavesPerBlock;
-    const int group = blockIdx.x / kGroup;
-    const int ngroups = (nb + kGroup - 1) / kGroup;
-    const unsigned gsize = (unsigned)min(kGroup, nb - group * kGroup);
+
+    // first step's plan (from k_init) and the problem states
+    for (int p = threadIdx.x; p < P; p += kBlock) {
+        const PlanEntry e = ws->plan[p];
+        s_flags[p] = e.flags;
+        s_alpha[p] = e.alpha;
+        s_beta[p] = e.beta;
+        s_toff[p] = e.tile_off;
+        s_coff[p] = e.chunk_off;
+        s_st[p] = S[p];
+    }
+    if (threadIdx.x == 0) {
+        s_hdr[0] = ws->n_tiles_active;
+        s_hdr[1] = ws->n_chunks_active;
+        s_hdr[2] = ws->any_active;
+    }
+    __syncthreads();
 
     for (int step = 0; step < max_steps; ++step) {
         const bool isA = (step & 1) == 0;
-        // ---- read the plan published for this step ----
+        const int par = step & 1, buf = step % 3;
+        if (!s_hdr[2]) break;
+        const int nT = s_hdr[0], nC = s_hdr[1];
         for (int p = threadIdx.x; p < P; p += kBlock) {
-            const PlanEntry e = ws->plan[p];
-            s_flags[p] = e.flags;
-            s_alpha[p] = e.alpha;
-            s_beta[p] = e.beta;
-            s_toff[p] = e.tile_off;
-            s_coff[p] = e.chunk_off;
             s_hi[p] = 0ull;
             s_lo[p] = 0ull;
             s_bad[p] = 0;
         }
-        if (threadIdx.x == 0) {
-            s_hdr[0] = ws->n_tiles_active;
-            s_hdr[1] = ws->n_chunks_active;
-            s_hdr[2] = ws->any_active;
-        }
         __syncthreads();
-        if (!s_hdr[2]) break;
-        const int nT = s_hdr[0], nC = s_hdr[1];
 
         // ---- node items, one block each (A: d, g.d, spring energy; B: gradient, dots) ----
         for (int c = (int)vb; c < nC; c += nb) {
@@ -703,9 +734,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
             const ProbDesc* D = descs + p;
             const int fl = s_flags[p];
             if (isA)
-                node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red);
+                node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par);
             else
-                node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red);
+                node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red, par);
         }
 
         // ---- tiles (A only): contiguous share per wave, software-pipelined:
@@ -797,45 +828,58 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
         if (isA) {
             const int slot = blockIdx.x % kSlots;
             for (int p = threadIdx.x; p < P; p += kBlock) {
-                if (s_hi[p]) atomicAdd(&ws->hi[p][slot], s_hi[p]);
-                if (s_lo[p]) atomicAdd(&ws->lo[p][slot], s_lo[p]);
-                if (s_bad[p]) atomicOr(&ws->bad[p], 1);
+                if (s_hi[p]) atomicAdd(&ws->hi[buf][p][slot], s_hi[p]);
+                if (s_lo[p]) atomicAdd(&ws->lo[buf][p][slot], s_lo[p]);
+                if (s_bad[p]) atomicOr(&ws->bad[buf][p], 1);
             }
         }
 
-        // ---- tree barrier; the last block to arrive runs the root step ----
+        // ---- grid barrier: every block publishes its arrival; block 0's first
+        //      wave waits for all of them and releases the step ----
         __syncthreads();
-        unsigned g = 0;
         if (threadIdx.x == 0) {
-            g = __hip_atomic_load(&ws->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            int root = 0;
-            const unsigned a = atomicAdd(&ws->group_cnt[group], 1u);
-            if (a == gsize - 1) {
-                atomicExch(&ws->group_cnt[group], 0u);
-                __threadfence();
-                const unsigned r = atomicAdd(&ws->root_cnt, 1u);
-                if (r == (unsigned)ngroups - 1) {
-                    atomicExch(&ws->root_cnt, 0u);
-                    root = 1;
-                }
-            }
-            __threadfence();
-            s_root = root;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             s_ok = 1;
         }
-        __syncthreads();
-        if (s_root) {
-            root_step(descs, P, S, ws, isA, s_flags, s_fin, s_sr2);
-            __syncthreads();
-            if (threadIdx.x == 0) {
+        if (blockIdx.x == 0) {
+            if (wave == 0) {
+                const uint64_t tstart = wall_clock64();
+                unsigned polls = 0;
+                int ok = 1;
+                while (true) {
+                    bool mine = true;
+                    for (int b = lane; b < nb; b += kWave)
+                        mine = mine && __hip_atomic_load(&ws->arrive[b * kArriveStride], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT) > (unsigned)step;
+                    if (__all(mine)) break;
+                    if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        ok = 0;
+                        break;
+                    }
+                    ++polls;
+                    if (polls > kBarrierMinPolls && wall_clock64() - tstart > kBarrierTimeoutTicks) {
+                        if (lane == 0) {
+                            atomicExch(&ws->err, 1);
+                            printf("d2ic: grid barrier timeout (arrivals, step %d)\n", step);
+                        }
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
                 __threadfence();
-                atomicAdd(&ws->gen, 1u);
+                for (int b = lane; b < nb; b += kWave)
+                    __hip_atomic_store(&ws->release[b * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0 && !ok) s_ok = 0;
             }
         } else if (threadIdx.x == 0) {
             const uint64_t tstart = wall_clock64();
             unsigned polls = 0;
-            while (__hip_atomic_load(&ws->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+            while (__hip_atomic_load(&ws->release[blockIdx.x * kArriveStride], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) <= (unsigned)step) {
                 if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     s_ok = 0;
                     break;
@@ -847,12 +891,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                     s_ok = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
             }
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) s_ok = 0;
         }
         __syncthreads();
         if (!s_ok) break;
+        block_decide(descs, P, S, ws, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_fin, s_sr2,
+                     s_badp);
     }
 }
 
@@ -1543,7 +1590,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     // split exact
     int64_t want = ceil_div(total_work, (int64_t)kWavesPerBlock * 4);
     want = std::max<int64_t>(8, ceil_div(want, 8) * 8);
-    int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kGroup * kMaxGroups), want);
+    int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kMaxBlocks), want);
     int max_steps = 8 + 2 * (2 + max_it * 12);
     void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps};
     d2ic_cg_plan* lead = plans[0];
