@@ -172,11 +172,10 @@ __device__ __forceinline__ int find_prob(const int* pref, int P, int item) {
     return lo;
 }
 
-// Per-tile record (32 floats, written by the B-step scatter and at start):
-// [a*6 + 0..5] = u, g, d_prev of local node a (a = 0..3), [24] = pixel count,
-// [25..28] = floor(min x) - 1, floor(max x), floor(min y) - 1, floor(max y)
-// over the tile's pixels (int bits).
-constexpr int kTV = 32;
+// Per-tile static record (8 ints, built with the plan): [0] pixel count,
+// [1..4] = floor(min x) - 1, floor(max x), floor(min y) - 1, floor(max y) over
+// the tile's pixels.
+constexpr int kTV = 8;
 constexpr int kLanesPerNode = 8;
 constexpr int kNodesPerWave = kWave / kLanesPerNode;
 constexpr int kNodesPerItem = kNodesPerWave * kWavesPerBlock;  // node item = one block
@@ -187,8 +186,30 @@ struct TilePix {  // per-lane pixel data of one chunk (f is loaded at use)
     float4 N;
 };
 
-__device__ __forceinline__ float tile_tv(const ProbDesc* __restrict__ D, int t, int lane) {
-    return lane < 29 ? gld(D->TV)[(size_t)t * kTV + lane] : 0.f;
+// A tile's wave-wide record, one value per lane: lanes 0..23 hold
+// (u, g, d_prev) of local node a = lane / 6 (component lane % 6), lanes
+// 24..28 the static record.  Built in two dependent loads: tile_ids gives each
+// lane its node id (or static field), tile_vals the value.
+typedef int i4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int tile_ids(const ProbDesc* __restrict__ D, int t, int lane) {
+    if (lane < 24) {
+        const i4v nd = gld(reinterpret_cast<const i4v*>(D->tile_nodes))[t];
+        const int a = lane / 6;
+        return a == 0 ? nd.x : (a == 1 ? nd.y : (a == 2 ? nd.z : nd.w));
+    }
+    if (lane < 29) return __float_as_int(gld(D->TV)[(size_t)t * kTV + (lane - 24)]);
+    return 0;
+}
+
+__device__ __forceinline__ float tile_vals(const ProbDesc* __restrict__ D, int id, int lane, int flags) {
+    if (lane < 24) {
+        const int c = lane % 6;
+        const float* base = c < 2 ? D->U[(flags & F_CUR) ? 1 : 0]
+                                  : (c < 4 ? D->G[(flags & F_CUR) ? 1 : 0] : D->D[(flags & F_DCUR) ? 1 : 0]);
+        return gld(base)[2 * (size_t)id + (c & 1)];
+    }
+    return __int_as_float(id);
 }
 
 __device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int t, int c, int lane) {
@@ -484,13 +505,6 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             gg = (double)gx * (double)gx + (double)gy * (double)gy;
             gnum = (double)gx * ((double)gx - (double)go.x) + (double)gy * ((double)gy - (double)go.y);
         }
-        for (int e = e0 + sub; e < e1; e += kLanesPerNode) {
-            const int ent = gld(D->node_ent)[e];
-            float* rec = D->TV + (size_t)(ent >> 2) * kTV + 6 * (ent & 3);
-            st2(rec, 0, un);
-            st2(rec, 1, make_float2(gx, gy));
-            st2(rec, 2, dn);
-        }
     }
     block_store2(gg, gnum, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
 }
@@ -754,36 +768,48 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                 pa = next_prob(pa, t0);
                 unsigned long long hi = 0ull, lo = 0ull;
                 int bad = 0;
-                // prologue: tile t0 record, geometry, box and first chunk; t0+1 record
+                // prologue: tile t0 resolved (record, geometry, box loads, first
+                // chunk); t0+1 values and t0+2 node ids in flight
                 float2 uc[4];
-                const float tv_cur = tile_tv(descs + pa, t0 - s_toff[pa], lane);
+                const float tv_cur =
+                    tile_vals(descs + pa, tile_ids(descs + pa, t0 - s_toff[pa], lane), lane, s_flags[pa]);
                 tile_uc(tv_cur, s_flags[pa], s_alpha[pa], s_beta[pa], uc);
                 TileBox b_cur = tile_box(tv_cur, uc);
                 float R[kBoxRegs];
                 box_load(descs + pa, b_cur, lane, R);
                 TilePix px = tile_pix(descs + pa, t0 - s_toff[pa], 0, lane);
-                int pb = pa;
-                float tv_nxt = 0.f;
+                int pb = pa, pc = pa;
+                float tv_n1 = 0.f;
+                int id_n2 = 0;
                 if (t0 + 1 < t1) {
                     pb = next_prob(pa, t0 + 1);
-                    tv_nxt = tile_tv(descs + pb, t0 + 1 - s_toff[pb], lane);
+                    tv_n1 = tile_vals(descs + pb, tile_ids(descs + pb, t0 + 1 - s_toff[pb], lane), lane, s_flags[pb]);
+                    pc = pb;
+                    if (t0 + 2 < t1) {
+                        pc = next_prob(pb, t0 + 2);
+                        id_n2 = tile_ids(descs + pc, t0 + 2 - s_toff[pc], lane);
+                    }
                 }
                 for (int it = t0; it < t1; ++it) {
                     const ProbDesc* Da = descs + pa;
                     const int ta = it - s_toff[pa];
-                    // this tile's box into LDS, then the next tile's geometry and box loads
+                    // this tile's box into LDS, then the next tile's geometry and box
+                    // loads; values of tile it+2 and node ids of tile it+3 requested
                     box_store(b_cur, lane, R, my_box);
                     TileBox b_nxt;
-                    float tv_nn = 0.f;
-                    int pc = pb;
                     if (it + 1 < t1) {
                         float2 ucn[4];
-                        tile_uc(tv_nxt, s_flags[pb], s_alpha[pb], s_beta[pb], ucn);
-                        b_nxt = tile_box(tv_nxt, ucn);
+                        tile_uc(tv_n1, s_flags[pb], s_alpha[pb], s_beta[pb], ucn);
+                        b_nxt = tile_box(tv_n1, ucn);
                         box_load(descs + pb, b_nxt, lane, R);
-                        if (it + 2 < t1) {
-                            pc = next_prob(pb, it + 2);
-                            tv_nn = tile_tv(descs + pc, it + 2 - s_toff[pc], lane);
+                    }
+                    float tv_n2 = 0.f;
+                    int id_n3 = 0, pd = pc;
+                    if (it + 2 < t1) {
+                        tv_n2 = tile_vals(descs + pc, id_n2, lane, s_flags[pc]);
+                        if (it + 3 < t1) {
+                            pd = next_prob(pc, it + 3);
+                            id_n3 = tile_ids(descs + pd, it + 3 - s_toff[pd], lane);
                         }
                     }
                     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -816,11 +842,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                         hi = lo = 0ull;
                         bad = 0;
                     }
-                    if (it + 1 < t1) tile_uc(tv_nxt, s_flags[pb], s_alpha[pb], s_beta[pb], uc);
+                    if (it + 1 < t1) tile_uc(tv_n1, s_flags[pb], s_alpha[pb], s_beta[pb], uc);
                     pa = pb;
                     pb = pc;
+                    pc = pd;
                     b_cur = b_nxt;
-                    tv_nxt = tv_nn;
+                    tv_n1 = tv_n2;
+                    id_n2 = id_n3;
                 }
             }
         }
@@ -925,16 +953,6 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
     if (D.hist)
         for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 3 * D.max_iter; k += gridDim.x * blockDim.x)
             D.hist[k] = NAN;
-    // tile records for the first pass: (u0, g = 0, d = 0) per tile node
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 4 * D.ntiles; e += gridDim.x * blockDim.x) {
-        const int t = e >> 2, a = e & 3;
-        const int4 nd = D.tile_nodes[t];
-        const int n = a == 0 ? nd.x : (a == 1 ? nd.y : (a == 2 ? nd.z : nd.w));
-        float* rec = D.TV + (size_t)t * kTV + 6 * a;
-        rec[0] = D.U[0][2 * n];
-        rec[1] = D.U[0][2 * n + 1];
-        rec[2] = rec[3] = rec[4] = rec[5] = 0.f;
-    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ProbState st;
         st.mode = MODE_INIT;
@@ -1075,11 +1093,11 @@ __global__ void k_tv_static(const int32_t* __restrict__ tcnt, int64_t nt, const 
         ymin = fminf(ymin, y);
         ymax = fmaxf(ymax, y);
     }
-    TV[t * kTV + 24] = __int_as_float(cnt);
-    TV[t * kTV + 25] = __int_as_float((int)floorf(xmin) - 1);
-    TV[t * kTV + 26] = __int_as_float((int)floorf(xmax));
-    TV[t * kTV + 27] = __int_as_float((int)floorf(ymin) - 1);
-    TV[t * kTV + 28] = __int_as_float((int)floorf(ymax));
+    TV[t * kTV + 0] = __int_as_float(cnt);
+    TV[t * kTV + 1] = __int_as_float((int)floorf(xmin) - 1);
+    TV[t * kTV + 2] = __int_as_float((int)floorf(xmax));
+    TV[t * kTV + 3] = __int_as_float((int)floorf(ymin) - 1);
+    TV[t * kTV + 4] = __int_as_float((int)floorf(ymax));
 }
 
 // Padded layout defaults: pixel centre (0.5, 0.5), zero shape functions.
