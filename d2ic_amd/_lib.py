@@ -44,6 +44,11 @@ _SIGS = {
     "d2ic_cg_solve_batch": [_i32, _pp, _pp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), _pp,
                             ctypes.POINTER(_f32), ctypes.POINTER(_i32), ctypes.POINTER(_f32), _pp, _pp, _pp, _vp],
     "d2ic_cg_kernel_ms": [_vp, ctypes.POINTER(_f32)],
+    "d2ic_serve_create": [_i32, ctypes.c_double, ctypes.POINTER(_vp)],
+    "d2ic_serve_submit": [_vp, _vp, _vp, _i32, _i32, _vp, _f32, _i32, _f32, _vp, _vp, _vp, ctypes.POINTER(_i32),
+                          ctypes.POINTER(_i64)],
+    "d2ic_serve_poll": [_vp, _i32, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    "d2ic_serve_destroy": [_vp],
     "d2ic_zncc_match": [_vp, _i64, _vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp],
     "d2ic_idw": [_vp, _i64, _vp, _vp, _i64, _vp, _vp],
     "d2ic_strain": [_vp, _vp, _i64, _vp, _vp, _i32, _f32, _f32, _vp, _vp, _vp, _vp],

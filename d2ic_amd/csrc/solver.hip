@@ -36,9 +36,12 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -54,7 +57,9 @@ constexpr int kMaxProblems = 256;
 constexpr int kSlots = 16;    // spread of the fixed-point residual sums
 constexpr double kFixScale = 1099511627776.0;  // 2^40
 
-enum Mode : int { MODE_INIT = 0, MODE_PASS = 1, MODE_NODE = 2, MODE_DONE = 3 };
+// MODE_COPY: solved with the iterate in the plan's buffer; one node pass copies
+// it to the caller's buffer before the problem is reported done (served solves).
+enum Mode : int { MODE_INIT = 0, MODE_PASS = 1, MODE_NODE = 2, MODE_DONE = 3, MODE_COPY = 4 };
 
 enum Flag : int {
     F_DOA = 1,
@@ -62,12 +67,14 @@ enum Flag : int {
     F_RESTART = 4,
     F_CUR = 16,
     F_DCUR = 32,
+    F_INIT = 64,   // first evaluation at u0: g and d_prev are not read (treated as 0)
+    F_COPY = 128,  // node pass copying U[1] to U[0]
 };
 
 struct ProbDesc {
-    const int32_t* tile_cnt;  // pixels in the tile (<= 64)
+    const int32_t* tile_cnt;  // pixels in the tile (<= kTile)
     const int4* tile_nodes;   // 4 node ids
-    const float* px;          // padded: pixel (t, lane) at t * 64 + lane
+    const float* px;          // padded: pixel i of tile t at t * kTile + i
     const float* py;
     const float4* N4;
     const float* F;
@@ -91,7 +98,8 @@ struct ProbDesc {
     double invNp;
 };
 
-// Solver state of one problem; read and written only by the root block.
+// Solver state of one problem.  Every block keeps an identical copy in LDS and
+// advances it identically; block 0 also stores it to global memory.
 struct ProbState {
     int mode, k, trial, restart, final_eval, cur, dcur, passes;
     float alpha, beta, gnorm, Jtrial;
@@ -110,6 +118,24 @@ struct PlanEntry {
 constexpr int kMaxBlocks = 1024;
 constexpr int kArriveStride = 32;  // one 128-byte line per block's arrival flag
 
+// Mailbox of a served session, in pinned host memory (see d2ic_serve_*).
+// The host fills desc[i], then raises submit_gen[i]; the kernel admits the
+// request at its next grid barrier (admitted_gen[i] = submit_gen[i]) and,
+// once solved, publishes done_iters/done_passes and then done_gen[i].
+constexpr int kMaxSlots = 64;
+struct ServeHost {
+    ProbDesc desc[kMaxSlots];
+    unsigned long long submit_gen[kMaxSlots];
+    unsigned long long admitted_gen[kMaxSlots];
+    unsigned long long done_gen[kMaxSlots];
+    int done_iters[kMaxSlots];
+    int done_passes[kMaxSlots];
+    unsigned long long doorbell;    // raised by the host after every submission
+    unsigned long long idle_ticks;  // the kernel exits after this long without work (wall-clock ticks)
+    int shutdown;                   // host: exit as soon as idle
+    int alive;                      // 1 while a served kernel runs (host sets it before launch)
+};
+
 // Per-launch workspace.  The residual accumulators are triple-buffered by
 // step (written during step s, read right after barrier s, cleared during
 // step s + 2 by block 0).
@@ -121,6 +147,8 @@ struct Workspace {
     int n_tiles_active, n_chunks_active, any_active, pad0;
     unsigned arrive[kMaxBlocks * kArriveStride];   // last step each block finished
     unsigned release[kMaxBlocks * kArriveStride];  // last step every block finished (one line per block)
+    unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
+    int stop;                                      // served mode: every block leaves after this barrier
     int err;
     int pad1[2];
 };
@@ -232,13 +260,13 @@ __device__ __forceinline__ float uni(float v) {  // wave-uniform value -> SGPR
 
 // Candidate node displacements of a tile (wave-uniform, from the tile record).
 __device__ __forceinline__ void tile_uc(float tv, int flags, float alpha, float beta, float2 (&uc)[4]) {
-    const bool restart = (flags & F_RESTART) != 0;
+    const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const float2 u = make_float2(bcast(tv, 6 * a), bcast(tv, 6 * a + 1));
         const float2 gr = make_float2(bcast(tv, 6 * a + 2), bcast(tv, 6 * a + 3));
         const float2 dp = make_float2(bcast(tv, 6 * a + 4), bcast(tv, 6 * a + 5));
-        const float2 c = step_to(u, direction(gr, dp, beta, restart), alpha);
+        const float2 c = init ? u : step_to(u, direction(gr, dp, beta, restart), alpha);
         uc[a] = make_float2(uni(c.x), uni(c.y));
     }
 }
@@ -435,8 +463,9 @@ __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags,
         const float* U = D->U[cur];
         const float* G = D->G[cur];
         const float* Dp = D->D[dcur];
-        const float2 g = ld2(G, n);
-        const float2 d = direction(g, ld2(Dp, n), beta, restart);
+        const bool init = (flags & F_INIT) != 0;
+        const float2 g = init ? make_float2(0.f, 0.f) : ld2(G, n);
+        const float2 d = init ? make_float2(0.f, 0.f) : direction(g, ld2(Dp, n), beta, restart);
         if (sub == 0) {
             st2(D->D[dcur ^ 1], n, d);
             gd = (double)g.x * (double)d.x + (double)g.y * (double)d.y;
@@ -447,12 +476,23 @@ __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags,
             const int j = gld(D->nb_idx)[(size_t)n * md + k];
             if (j < 0) continue;
             const float w = gld(D->nb_wp)[(size_t)n * md + k];
-            const float2 ucj = step_to(ld2(U, j), direction(ld2(G, j), ld2(Dp, j), beta, restart), alpha);
+            const float2 ucj =
+                init ? ld2(U, j) : step_to(ld2(U, j), direction(ld2(G, j), ld2(Dp, j), beta, restart), alpha);
             const float dx = ucn.x - ucj.x, dy = ucn.y - ucj.y;
             E += (double)(w * (dx * dx + dy * dy));
         }
     }
     block_store2(gd, E, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
+}
+
+// Node pass of a finished served problem: copy the iterate from the plan's
+// buffer U[1] to the caller's buffer U[0].
+__device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
+    const int n0 = item * kNodesPerItem;
+    for (int i = threadIdx.x; i < 2 * kNodesPerItem; i += kBlock) {
+        const int n = n0 + i / 2;
+        if (n < D->nnodes) gst(D->U[0])[2 * (size_t)n + (i & 1)] = gld(D->U[1])[2 * (size_t)n + (i & 1)];
+    }
 }
 
 // B-step node item: new point, gradient gather, |g|^2, g.(g - g_prev), and
@@ -592,6 +632,7 @@ __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double 
     if (st.restart) flags |= F_RESTART;
     if (st.cur) flags |= F_CUR;
     if (st.dcur) flags |= F_DCUR;
+    if (st.mode == MODE_INIT) flags |= F_INIT;
     return flags;
 }
 
@@ -600,7 +641,7 @@ __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double 
 // of the problem states, and derive the next step's plan (identical in every
 // block).  Block 0 alone writes histories, n_iters and the global states, and
 // clears the accumulators of step + 2.
-__device__ void block_decide(const ProbDesc* __restrict__ descs, int P, ProbState* __restrict__ S, Workspace* ws,
+__device__ void block_decide(const ProbDesc* descs, int P, ProbState* __restrict__ S, Workspace* ws, ServeHost* sh,
                              int step, ProbState* s_st, int* s_flags, float* s_alpha, float* s_beta, int* s_toff,
                              int* s_coff, int* s_hdr, double (*s_fin)[2], double* s_sr2, int* s_badp) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -642,9 +683,51 @@ __device__ void block_decide(const ProbDesc* __restrict__ descs, int P, ProbStat
         }
     }
     __syncthreads();
+    const unsigned long long admit =
+        sh ? __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
     for (int p = threadIdx.x; p < P; p += kBlock) {
         ProbState st = s_st[p];
-        const int fl = decide(descs[p], st, just_did_a, s_sr2[p], s_badp[p] != 0, s_fin[p][0], s_fin[p][1], writer);
+        const int prev = st.mode;
+        bool finished = false;
+        int fl;
+        if (prev == MODE_COPY) {  // the copy pass ran in this step
+            st.mode = MODE_DONE;
+            fl = 0;
+            finished = true;
+        } else {
+            fl = decide(descs[p], st, just_did_a, s_sr2[p], s_badp[p] != 0, s_fin[p][0], s_fin[p][1], writer);
+            if (prev != MODE_DONE && st.mode == MODE_DONE) {
+                if (sh && st.cur) {
+                    st.mode = MODE_COPY;
+                    fl = F_COPY;
+                } else {
+                    finished = true;
+                }
+            }
+        }
+        if (sh && ((admit >> p) & 1ull)) {
+            // a served request joins: first evaluation at u0 on the next A-step
+            ProbState z;
+            z.mode = MODE_INIT;
+            z.k = z.trial = z.restart = z.final_eval = z.cur = z.dcur = 0;
+            z.alpha = z.beta = z.gnorm = z.Jtrial = 0.f;
+            z.J = z.gg = 0.0;
+            const bool next_is_a = just_did_a ? false : true;
+            z.passes = next_is_a ? 1 : 0;
+            fl = next_is_a ? (F_DOA | F_INIT) : 0;
+            st = z;
+            finished = false;
+        }
+        if (finished && writer && sh) {
+            const ProbDesc& D = descs[p];
+            if (D.pass_count) *D.pass_count = st.passes;
+            sh->done_iters[p] = st.k;
+            sh->done_passes[p] = st.passes;
+            __threadfence_system();
+            __hip_atomic_store(&sh->done_gen[p],
+                               __hip_atomic_load(&sh->admitted_gen[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
         s_st[p] = st;
         if (writer) S[p] = st;
         s_flags[p] = fl;
@@ -668,7 +751,7 @@ __device__ void block_decide(const ProbDesc* __restrict__ descs, int P, ProbStat
             s_toff[p] = to;
             s_coff[p] = co;
             if (fl & F_DOA) to += descs[p].ntiles;
-            if (fl & (F_DOA | F_DOB)) co += descs[p].nchunks;
+            if (fl & (F_DOA | F_DOB | F_COPY)) co += descs[p].nchunks;
             if (s_st[p].mode != MODE_DONE) any = 1;  // a problem whose trial was rejected idles one step
         }
         s_hdr[0] = to;
@@ -678,14 +761,82 @@ __device__ void block_decide(const ProbDesc* __restrict__ descs, int P, ProbStat
     __syncthreads();
 }
 
+// Served mode, master wave (block 0, wave 0), after every block has arrived:
+// admit the requests waiting in the mailbox into free slots (their
+// descriptors are copied to the device slot table) and decide whether the
+// session ends.  Results go to ws->admit_mask / ws->stop, which every block
+// reads after the barrier.  A slot is free when its problem is done in the
+// (identical) block states.
+__device__ void serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh, const ProbState* s_st,
+                            bool any_active, unsigned long long bell, int lane, unsigned long long& seen_bell,
+                            uint64_t& idle_t0) {
+    unsigned long long mask = 0ull;
+    bool pending = false;
+    if (bell != seen_bell) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        bool take = false;
+        unsigned long long g = 0ull;
+        if (lane < P) {
+            g = __hip_atomic_load(&sh->submit_gen[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            const unsigned long long a =
+                __hip_atomic_load(&sh->admitted_gen[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (g > a) {
+                if (s_st[lane].mode == MODE_DONE)
+                    take = true;
+                else
+                    pending = true;
+            }
+        }
+        mask = __ballot(take);
+        pending = __any(pending);
+        // copy each admitted descriptor, one 8-byte word per lane
+        constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
+        static_assert(sizeof(ProbDesc) % 8 == 0 && kWords <= kWave, "descriptor copy layout");
+        unsigned long long m = mask;
+        while (m) {
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            if (lane < kWords) {
+                const unsigned long long w = __hip_atomic_load(
+                    reinterpret_cast<unsigned long long*>(&sh->desc[i]) + lane, __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_SYSTEM);
+                reinterpret_cast<unsigned long long*>(&descs[i])[lane] = w;
+            }
+        }
+        if (take) __hip_atomic_store(&sh->admitted_gen[lane], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (!pending) seen_bell = bell;  // rescan later while a request waits for its slot
+        __threadfence();
+    }
+    const uint64_t now = wall_clock64();
+    if (any_active || mask || pending) idle_t0 = now;
+    const bool stop = !any_active && !mask && !pending &&
+                      (__hip_atomic_load(&sh->shutdown, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
+                       now - idle_t0 > sh->idle_ticks);
+    if (lane == 0) {
+        __hip_atomic_store(&ws->admit_mask, mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ws->stop, stop ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+}
+
 // wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
 // a wait is abandoned only after both 2^20 polls and 4 s.
 constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
 constexpr unsigned kBarrierMinPolls = 1u << 20;
 
-__global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __restrict__ descs, int P,
-                                                          ProbState* __restrict__ S, Workspace* __restrict__ ws,
-                                                          int max_steps) {
+// sh == nullptr: a fixed batch of P problems (d2ic_cg_solve_batch).  Otherwise a
+// served session over P <= kMaxSlots slots whose requests arrive through the
+// mailbox; the kernel runs until it has been idle for sh->idle_ticks or the
+// host asks it to shut down.
+// The fixed-batch instance takes the descriptor table as const __restrict__
+// (its fields become scalar loads); the served instance rewrites slots while
+// it runs, so its table is an ordinary pointer.
+template <bool kServe>
+using DescTable = typename std::conditional<kServe, ProbDesc*, const ProbDesc* __restrict__>::type;
+
+template <bool kServe>
+__global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> descs, int P, ProbState* __restrict__ S,
+                                                          Workspace* __restrict__ ws, int max_steps, ServeHost* sh) {
     __shared__ int s_flags[kMaxProblems];
     __shared__ float s_alpha[kMaxProblems];
     __shared__ float s_beta[kMaxProblems];
@@ -730,10 +881,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
     }
     __syncthreads();
 
+    unsigned long long seen_bell = ~0ull;  // served mode, master wave: last doorbell scanned
+    uint64_t idle_t0 = wall_clock64();     // served mode, master wave: start of the current idle spell
+
     for (int step = 0; step < max_steps; ++step) {
         const bool isA = (step & 1) == 0;
         const int par = step & 1, buf = step % 3;
-        if (!s_hdr[2]) break;
+        if (!sh && !s_hdr[2]) break;
         const int nT = s_hdr[0], nC = s_hdr[1];
         for (int p = threadIdx.x; p < P; p += kBlock) {
             s_hi[p] = 0ull;
@@ -747,7 +901,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
             const int p = find_prob(s_coff, P, c);
             const ProbDesc* D = descs + p;
             const int fl = s_flags[p];
-            if (isA)
+            if (fl & F_COPY)
+                node_item_copy(D, c - s_coff[p]);
+            else if (isA)
                 node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par);
             else
                 node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red, par);
@@ -876,6 +1032,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                 const uint64_t tstart = wall_clock64();
                 unsigned polls = 0;
                 int ok = 1;
+                // served mode: read the doorbell while the arrivals come in
+                unsigned long long bell = 0ull;
+                if (sh) bell = __hip_atomic_load(&sh->doorbell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 while (true) {
                     bool mine = true;
                     for (int b = lane; b < nb; b += kWave)
@@ -898,6 +1057,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
                     __builtin_amdgcn_s_sleep(1);
                 }
                 __threadfence();
+                if constexpr (kServe)
+                    serve_admit(descs, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
                 for (int b = lane; b < nb; b += kWave)
                     __hip_atomic_store(&ws->release[b * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -926,8 +1087,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(const ProbDesc* __rest
         }
         __syncthreads();
         if (!s_ok) break;
-        block_decide(descs, P, S, ws, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_fin, s_sr2,
+        if (sh && __hip_atomic_load(&ws->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        block_decide(descs, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_fin, s_sr2,
                      s_badp);
+    }
+    if (sh && blockIdx.x == 0 && threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(&sh->alive, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -972,7 +1138,7 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
             co += descs[q].nchunks;
         }
         PlanEntry e;
-        e.flags = F_DOA;
+        e.flags = F_DOA | F_INIT;
         e.alpha = 0.f;
         e.beta = 0.f;
         e.tile_off = to;
@@ -985,6 +1151,18 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
             ws->any_active = 1;
         }
     }
+}
+
+// Served session start: every slot idle (done).
+__global__ void k_serve_init(ProbState* __restrict__ S, int P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    ProbState st;
+    st.mode = MODE_DONE;
+    st.k = st.trial = st.restart = st.final_eval = st.cur = st.dcur = st.passes = 0;
+    st.alpha = st.beta = st.gnorm = st.Jtrial = 0.f;
+    st.J = st.gg = 0.0;
+    S[p] = st;
 }
 
 // Copy the final iterate into the caller's buffer (U[0]) and publish counters;
@@ -1194,6 +1372,24 @@ struct d2ic_cg_plan {
     bool timed = false;
 };
 
+// A served session: the mailbox (pinned), the device slot table and the
+// workspace of the persistent kernel, which runs on the session's own stream
+// while requests are in flight.
+struct d2ic_server {
+    int device = 0;
+    int slots = 0;
+    hipStream_t stream = nullptr;
+    ServeHost* sh = nullptr;
+    ProbDesc* d_desc = nullptr;
+    ProbState* d_S = nullptr;
+    Workspace* d_ws = nullptr;
+    std::mutex mu;
+    bool launched = false;
+    unsigned long long gen = 0;
+    std::vector<unsigned long long> slot_gen;  // request in the slot (0: free)
+    std::vector<const d2ic_cg_plan*> slot_plan;
+};
+
 namespace {
 
 template <class T>
@@ -1233,15 +1429,17 @@ struct Staging {
 };
 thread_local Staging g_stage;
 
-int occupancy_blocks(int* out) {
+// Co-resident capacity of a persistent-kernel instance (cached per device).
+int occupancy_blocks(const void* kernel, int* out) {
     static std::mutex mu;
-    static int cached_dev = -1, cached = 0;
+    static int cached_dev[2] = {-1, -1}, cached[2] = {0, 0};
+    const int slot = kernel == (const void*)k_cg_persistent<true> ? 1 : 0;
     int dev = 0;
     D2IC_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(mu);
-    if (dev != cached_dev) {
+    if (dev != cached_dev[slot]) {
         int per_cu = 0;
-        D2IC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_persistent, kBlock, 0));
+        D2IC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
         hipDeviceProp_t prop;
         D2IC_HIP(hipGetDeviceProperties(&prop, dev));
         int coop = 0;
@@ -1250,11 +1448,55 @@ int occupancy_blocks(int* out) {
             set_error("device %d does not support cooperative launches", dev);
             return D2IC_ERR_DEVICE;
         }
-        cached = std::max(1, per_cu) * prop.multiProcessorCount;
-        cached_dev = dev;
+        cached[slot] = std::max(1, per_cu) * prop.multiProcessorCount;
+        cached_dev[slot] = dev;
     }
-    *out = cached;
+    *out = cached[slot];
     return D2IC_OK;
+}
+
+}  // namespace
+
+namespace {
+
+ProbDesc make_desc(const d2ic_cg_plan* pl, const float* def_image, int32_t H, int32_t W, float* u, float alpha_reg,
+                   int32_t max_iter, float tol, float* hist, int32_t* n_iters, int32_t* pass_count) {
+    ProbDesc d;
+    d.tile_cnt = pl->tile_cnt;
+    d.tile_nodes = pl->tile_nodes;
+    d.px = pl->px;
+    d.py = pl->py;
+    d.N4 = pl->N4;
+    d.F = pl->F;
+    d.node_ptr = pl->node_ptr;
+    d.node_ent = pl->node_ent;
+    d.nb_idx = pl->nb_idx;
+    d.nb_wp = pl->nb_wp;
+    d.U[0] = u;
+    d.U[1] = pl->U1;
+    d.G[0] = pl->G[0];
+    d.G[1] = pl->G[1];
+    d.D[0] = pl->D[0];
+    d.D[1] = pl->D[1];
+    d.TP = pl->TP;
+    d.TV = pl->TV;
+    d.NCP = pl->NCP;
+    d.img = def_image;
+    d.hist = hist;
+    d.n_iters = n_iters;
+    d.pass_count = pass_count;
+    d.H = H;
+    d.W = W;
+    d.interp = pl->interp;
+    d.nb_maxdeg = pl->nb_maxdeg;
+    d.ntiles = (int)pl->ntiles;
+    d.nnodes = (int)pl->nn;
+    d.nchunks = (int)pl->nchunks;
+    d.max_iter = max_iter;
+    d.alpha_reg = alpha_reg;
+    d.tol = tol;
+    d.invNp = 1.0 / (double)(pl->np > 0 ? pl->np : 1);
+    return d;
 }
 
 }  // namespace
@@ -1328,7 +1570,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
     PA(pl->G[1], 2 * n_nodes);
     PA(pl->D[0], 2 * n_nodes);
     PA(pl->D[1], 2 * n_nodes);
-    PA(pl->NCP, 2 * pl->nchunks);
+    PA(pl->NCP, 4 * pl->nchunks);  // 2 doubles per chunk, double-buffered by step parity
     PA(pl->node_ptr, n_nodes + 1);
     PA(pl->nb_idx, n_nodes * nb_max_degree);
     PA(pl->nb_wp, n_nodes * nb_max_degree);
@@ -1520,41 +1762,8 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         D2IC_CHECK_ARG(u[p] != nullptr || pl->nn == 0, "u[%d] must not be NULL", p);
         D2IC_CHECK_ARG(n_iters[p] != nullptr, "n_iters[%d] must not be NULL", p);
         D2IC_CHECK_ARG(max_iter[p] >= 0 && max_iter[p] <= (1 << 20), "max_iter[%d] out of range", p);
-        ProbDesc& d = hd[p];
-        d.tile_cnt = pl->tile_cnt;
-        d.tile_nodes = pl->tile_nodes;
-        d.px = pl->px;
-        d.py = pl->py;
-        d.N4 = pl->N4;
-        d.F = pl->F;
-        d.node_ptr = pl->node_ptr;
-        d.node_ent = pl->node_ent;
-        d.nb_idx = pl->nb_idx;
-        d.nb_wp = pl->nb_wp;
-        d.U[0] = u[p];
-        d.U[1] = pl->U1;
-        d.G[0] = pl->G[0];
-        d.G[1] = pl->G[1];
-        d.D[0] = pl->D[0];
-        d.D[1] = pl->D[1];
-        d.TP = pl->TP;
-        d.TV = pl->TV;
-        d.NCP = pl->NCP;
-        d.img = def_images[p];
-        d.hist = hist ? hist[p] : nullptr;
-        d.n_iters = n_iters[p];
-        d.pass_count = pass_count ? pass_count[p] : nullptr;
-        d.H = H[p];
-        d.W = W[p];
-        d.interp = pl->interp;
-        d.nb_maxdeg = pl->nb_maxdeg;
-        d.ntiles = (int)pl->ntiles;
-        d.nnodes = (int)pl->nn;
-        d.nchunks = (int)pl->nchunks;
-        d.max_iter = max_iter[p];
-        d.alpha_reg = alpha_reg[p];
-        d.tol = tol[p];
-        d.invNp = 1.0 / (double)(pl->np > 0 ? pl->np : 1);
+        hd[p] = make_desc(pl, def_images[p], H[p], W[p], u[p], alpha_reg[p], max_iter[p], tol[p],
+                          hist ? hist[p] : nullptr, n_iters[p], pass_count ? pass_count[p] : nullptr);
         max_it = std::max(max_it, max_iter[p]);
         total_items += pl->ntiles + pl->nchunks;
         total_work += pl->ntiles * kChunks + pl->nchunks;
@@ -1601,7 +1810,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     D2IC_LAUNCH_CHECK();
 
     int coop_max = 0;
-    int rc = occupancy_blocks(&coop_max);
+    int rc = occupancy_blocks((const void*)k_cg_persistent<false>, &coop_max);
     if (rc != D2IC_OK) return rc;
     // enough blocks for ~4 chunks of 64 pixels per wave, at least one XCD-round
     // (8), at most what can be co-resident; a multiple of 8 keeps the XCD-aware
@@ -1610,7 +1819,8 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     want = std::max<int64_t>(8, ceil_div(want, 8) * 8);
     int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kMaxBlocks), want);
     int max_steps = 8 + 2 * (2 + max_it * 12);
-    void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps};
+    ServeHost* no_sh = nullptr;
+    void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps, (void*)&no_sh};
     d2ic_cg_plan* lead = plans[0];
     if (!lead->ev_start) D2IC_HIP(hipEventCreate(&lead->ev_start));
     if (!lead->ev_end) D2IC_HIP(hipEventCreate(&lead->ev_end));
@@ -1624,10 +1834,11 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         return e && e[0] == '1';
     }();
     if (plain_launch) {
-        k_cg_persistent<<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps);
+        k_cg_persistent<false><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
         D2IC_LAUNCH_CHECK();
     } else {
-        D2IC_HIP(hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(nblocks), dim3(kBlock), args, 0, st));
+        D2IC_HIP(hipLaunchCooperativeKernel((const void*)k_cg_persistent<false>, dim3(nblocks), dim3(kBlock), args, 0,
+                                            st));
     }
     D2IC_HIP(hipEventRecord(lead->ev_end, st));
     lead->timed = true;
@@ -1645,6 +1856,162 @@ int d2ic_cg_solve(d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t
     int32_t* ns[1] = {n_iters};
     int32_t* pc[1] = {pass_count};
     return d2ic_cg_solve_batch(1, plans, imgs, &H, &W, us, &alpha_reg, &max_iter, &tol, hs, ns, pc, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Served sessions (continuous batching)
+// ---------------------------------------------------------------------------
+static int serve_launch(d2ic_server* srv) {
+    if (srv->launched) {
+        // the previous kernel has announced its exit; wait for it to end
+        D2IC_HIP(hipStreamSynchronize(srv->stream));
+    }
+    int coop_max = 0;
+    int rc = occupancy_blocks((const void*)k_cg_persistent<true>, &coop_max);
+    if (rc != D2IC_OK) return rc;
+    // leave one CU per XCD free: the kernel occupies a CU's registers entirely,
+    // and the clients' own small kernels (buffer fills, copies) must still run
+    const int nblocks = std::max(8, ((std::min(coop_max, kMaxBlocks) - 8) / 8) * 8);
+    k_zero_words<<<64, 256, 0, srv->stream>>>(reinterpret_cast<uint32_t*>(srv->d_ws),
+                                               (int64_t)(sizeof(Workspace) / 4));
+    D2IC_LAUNCH_CHECK();
+    k_serve_init<<<1, kMaxSlots, 0, srv->stream>>>(srv->d_S, srv->slots);
+    D2IC_LAUNCH_CHECK();
+    __atomic_store_n(&srv->sh->alive, 1, __ATOMIC_RELEASE);
+    int P = srv->slots;
+    int max_steps = INT32_MAX;
+    void* args[] = {(void*)&srv->d_desc, (void*)&P, (void*)&srv->d_S, (void*)&srv->d_ws, (void*)&max_steps,
+                    (void*)&srv->sh};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_cg_persistent<true>, dim3(nblocks), dim3(kBlock), args, 0,
+                                              srv->stream);
+    if (e != hipSuccess) {
+        __atomic_store_n(&srv->sh->alive, 0, __ATOMIC_RELEASE);
+        set_error("served kernel launch failed: %s", hipGetErrorString(e));
+        return D2IC_ERR_DEVICE;
+    }
+    srv->launched = true;
+    return D2IC_OK;
+}
+
+// Relaunch when the kernel has left while a submitted request was not admitted.
+static int serve_ensure(d2ic_server* srv) {
+    if (srv->launched && __atomic_load_n(&srv->sh->alive, __ATOMIC_ACQUIRE)) return D2IC_OK;
+    bool waiting = !srv->launched;
+    for (int i = 0; i < srv->slots && !waiting; ++i)
+        if (srv->slot_gen[i] && __atomic_load_n(&srv->sh->admitted_gen[i], __ATOMIC_ACQUIRE) < srv->slot_gen[i])
+            waiting = true;
+    return waiting ? serve_launch(srv) : D2IC_OK;
+}
+
+int d2ic_serve_create(int32_t slots, double idle_ms, d2ic_server** out) {
+    D2IC_CHECK_ARG(out != nullptr, "out must not be NULL");
+    D2IC_CHECK_ARG(slots >= 1 && slots <= kMaxSlots, "slots must be in [1, %d]", kMaxSlots);
+    D2IC_CHECK_ARG(idle_ms >= 0.0 && idle_ms <= 60000.0, "idle_ms must be in [0, 60000]");
+    auto* srv = new d2ic_server();
+    srv->slots = slots;
+    srv->slot_gen.assign(slots, 0ull);
+    srv->slot_plan.assign(slots, nullptr);
+#define SH(call)                                                                                   \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess) {                                                                    \
+            set_error("%s failed: %s", #call, hipGetErrorString(_e));                            \
+            d2ic_serve_destroy(srv);                                                               \
+            return D2IC_ERR_DEVICE;                                                                \
+        }                                                                                          \
+    } while (0)
+    SH(hipGetDevice(&srv->device));
+    SH(hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking));
+    SH(hipHostMalloc((void**)&srv->sh, sizeof(ServeHost), hipHostMallocCoherent));
+    memset(srv->sh, 0, sizeof(ServeHost));
+    srv->sh->idle_ticks = (unsigned long long)(idle_ms * 1e5);  // 100 MHz wall clock
+    SH(hipMalloc((void**)&srv->d_desc, sizeof(ProbDesc) * kMaxSlots));
+    SH(hipMalloc((void**)&srv->d_S, sizeof(ProbState) * kMaxSlots));
+    SH(hipMalloc((void**)&srv->d_ws, sizeof(Workspace)));
+#undef SH
+    *out = srv;
+    return D2IC_OK;
+}
+
+int d2ic_serve_submit(d2ic_server* srv, d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W, float* u,
+                      float alpha_reg, int32_t max_iter, float tol, float* hist, int32_t* n_iters,
+                      int32_t* pass_count, int32_t* slot_out, int64_t* ticket_out) {
+    D2IC_CHECK_ARG(srv && plan && slot_out && ticket_out, "server, plan, slot_out and ticket_out must not be NULL");
+    D2IC_CHECK_ARG(def_image != nullptr && H > 0 && W > 0, "deformed image must be non-empty");
+    D2IC_CHECK_ARG(u != nullptr || plan->nn == 0, "u must not be NULL");
+    D2IC_CHECK_ARG(n_iters != nullptr, "n_iters must not be NULL");
+    D2IC_CHECK_ARG(max_iter >= 0 && max_iter <= (1 << 20), "max_iter out of range");
+    if (!plan->has_ref) {
+        set_error("plan has no reference image (call d2ic_cg_set_reference first)");
+        return D2IC_ERR_STATE;
+    }
+    std::lock_guard<std::mutex> lk(srv->mu);
+    int slot = -1;
+    for (int i = 0; i < srv->slots; ++i) {
+        if (srv->slot_gen[i] && srv->slot_plan[i] == plan) {
+            set_error("plan already has a request in flight (slot %d)", i);
+            return D2IC_ERR_STATE;
+        }
+        if (!srv->slot_gen[i] && slot < 0) slot = i;
+    }
+    *slot_out = slot;
+    *ticket_out = 0;
+    if (slot < 0) return D2IC_OK;  // every slot busy: retry after a poll frees one
+    srv->sh->desc[slot] = make_desc(plan, def_image, H, W, u, alpha_reg, max_iter, tol, hist, n_iters, pass_count);
+    const unsigned long long g = ++srv->gen;
+    __atomic_store_n(&srv->sh->submit_gen[slot], g, __ATOMIC_RELEASE);
+    __atomic_fetch_add(&srv->sh->doorbell, 1ull, __ATOMIC_RELEASE);
+    srv->slot_gen[slot] = g;
+    srv->slot_plan[slot] = plan;
+    *ticket_out = (int64_t)g;
+    return serve_ensure(srv);
+}
+
+int d2ic_serve_poll(d2ic_server* srv, int32_t slot, int64_t ticket, int32_t* done, int32_t* n_iters,
+                    int32_t* passes) {
+    D2IC_CHECK_ARG(srv && done, "server and done must not be NULL");
+    D2IC_CHECK_ARG(slot >= 0 && slot < srv->slots, "slot out of range");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    *done = 0;
+    if (srv->slot_gen[slot] != (unsigned long long)ticket) {
+        set_error("slot %d does not hold ticket %lld", slot, (long long)ticket);
+        return D2IC_ERR_INVALID;
+    }
+    if (__atomic_load_n(&srv->sh->done_gen[slot], __ATOMIC_ACQUIRE) >= (unsigned long long)ticket) {
+        *done = 1;
+        if (n_iters) *n_iters = srv->sh->done_iters[slot];
+        if (passes) *passes = srv->sh->done_passes[slot];
+        srv->slot_gen[slot] = 0;
+        srv->slot_plan[slot] = nullptr;
+        return D2IC_OK;
+    }
+    if (srv->launched && !__atomic_load_n(&srv->sh->alive, __ATOMIC_ACQUIRE) &&
+        __atomic_load_n(&srv->sh->admitted_gen[slot], __ATOMIC_ACQUIRE) >= (unsigned long long)ticket) {
+        // admitted but the kernel left before finishing it: a barrier wait expired
+        hipError_t e = hipStreamSynchronize(srv->stream);
+        set_error("served kernel ended before request %lld finished (%s)", (long long)ticket,
+                  e == hipSuccess ? "grid barrier timeout" : hipGetErrorString(e));
+        srv->slot_gen[slot] = 0;
+        srv->slot_plan[slot] = nullptr;
+        return D2IC_ERR_TIMEOUT;
+    }
+    return serve_ensure(srv);
+}
+
+int d2ic_serve_destroy(d2ic_server* srv) {
+    if (!srv) return D2IC_OK;
+    if (srv->sh) {
+        __atomic_store_n(&srv->sh->shutdown, 1, __ATOMIC_RELEASE);
+        __atomic_fetch_add(&srv->sh->doorbell, 1ull, __ATOMIC_RELEASE);
+    }
+    if (srv->stream) (void)hipStreamSynchronize(srv->stream);
+    if (srv->d_ws) (void)hipFree(srv->d_ws);
+    if (srv->d_S) (void)hipFree(srv->d_S);
+    if (srv->d_desc) (void)hipFree(srv->d_desc);
+    if (srv->sh) (void)hipHostFree(srv->sh);
+    if (srv->stream) (void)hipStreamDestroy(srv->stream);
+    delete srv;
+    return D2IC_OK;
 }
 
 }  // extern "C"

@@ -119,6 +119,37 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
 int d2ic_cg_kernel_ms(const d2ic_cg_plan* plan, float* ms);
 
 /* ---------------------------------------------------------------------------
+ * Served sessions: continuous batching of solves in one persistent kernel.
+ * No reference counterpart (the reference solves one frame per call,
+ * solver_global_cg.py:155-198); each request is the same solve as
+ * d2ic_cg_solve and gives bitwise the same result.
+ * ------------------------------------------------------------------------ */
+typedef struct d2ic_server d2ic_server;
+
+/* Create a session with `slots` (1..64) concurrent requests.  The kernel runs
+ * on the session's own stream while requests are in flight and leaves after
+ * idle_ms without work (it is relaunched by the next submit or poll). */
+int d2ic_serve_create(int32_t slots, double idle_ms, d2ic_server** server);
+
+/* Queue one solve (same arguments as d2ic_cg_solve).  Inputs must be ready on
+ * the device (synchronise their producer first); hist, if given, must already
+ * hold NaN.  The request joins the running kernel at its next grid barrier.
+ * *slot = -1 (and *ticket = 0) when every slot is busy: poll, then retry.
+ * A plan may have one request in flight at a time.  Thread-safe. */
+int d2ic_serve_submit(d2ic_server* server, d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W,
+                      float* u, float alpha_reg, int32_t max_iter, float tol, float* hist, int32_t* n_iters,
+                      int32_t* pass_count, int32_t* slot, int64_t* ticket);
+
+/* Non-blocking completion check of (slot, ticket).  *done = 1 once u, hist,
+ * n_iters and pass_count hold the result; the slot is then free.  n_iters
+ * and passes (may be NULL) receive host copies.  Thread-safe. */
+int d2ic_serve_poll(d2ic_server* server, int32_t slot, int64_t ticket, int32_t* done, int32_t* n_iters,
+                    int32_t* passes);
+
+/* Finish outstanding work, stop the kernel and free the session. */
+int d2ic_serve_destroy(d2ic_server* server);
+
+/* ---------------------------------------------------------------------------
  * Translation ZNCC (replaces _match_centers_jax, solver_translation_zncc.py:118-168)
  * ------------------------------------------------------------------------ */
 /* centers (n,2) f32; ref (Hr,Wr) and def (Hd,Wd) f32 row-major.  u_out (n,2)
