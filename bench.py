@@ -45,6 +45,251 @@ def build_workload(n_frames: int):
     return ref, frames, mask
 
 
+def _zncc_init(mesh, assets, ref, de, search=4, win=41, max_centres=4096):
+    """ZNCC shifts at (a regular subsample of) the element centres, IDW to the
+    nodes: the initial guess shared by the plain and the batched/served paths."""
+    from d2ic_amd.dic_init_motion import project_centers_to_nodes_idw
+    from d2ic_amd.solver_translation_zncc import match_centers
+
+    c = np.asarray(assets.element_centers_xy, dtype=np.float32)
+    step = max(1, int(np.ceil(len(c) / max_centres)))
+    c = c[::step]
+    u, s = match_centers(c, ref, de, win=win, search=search)
+    return project_centers_to_nodes_idw(mesh.nodes_xy, c, u)
+
+
+def _problem(rng, H, e, roi, A, b, amp, wl, seed):
+    """One frame pair: speckle of the given size, ROI, Q4 mesh and plan."""
+    import torch
+
+    from d2ic_amd import speckle
+    from d2ic_amd.mask2mesh import mask_to_mesh
+    from d2ic_amd.mesh_assets import compute_element_centers
+    from d2ic_amd.pixel_assets import build_pixel_assets
+    from d2ic_amd.solver_global_cg import CGPlan
+
+    W = H
+    if roi == "hole":
+        mask = speckle.plate_with_hole_mask(H, W, int(0.88 * H), 0.15 * H)
+    else:
+        mask = speckle.rect_mask(H, W, int(0.8 * H), int(0.8 * H))
+    mesh = mask_to_mesh(mask, e)
+    fn = speckle.affine_field(A, b, amp, wl, W / 2.0, H / 2.0)
+    ref, frames = speckle.speckle_sequence_device(H, W, seed, [fn], noise_sigma=1.0)
+    pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=mask)
+    plan = CGPlan(pa)
+    plan.set_reference(ref, "cubic")
+
+    class _A:
+        element_centers_xy = compute_element_centers(mesh)
+
+    u0 = _zncc_init(mesh, _A, ref, frames[0])
+    torch.cuda.synchronize()
+    return dict(H=H, e=e, plan=plan, img=frames[0], u0=u0, np=plan.n_pixels, nn=plan.n_nodes, ref=ref)
+
+
+def _random_motion(rng, H, max_grad=5e-3, max_amp=3.0):
+    A = rng.uniform(-max_grad, max_grad, size=(2, 2)) / 2.0
+    half = H * 0.45
+    lin = np.abs(A).sum(axis=1).max() * half
+    amp = rng.uniform(0.2, 1.0)
+    bmax = max(0.0, max_amp - lin - amp)
+    b = rng.uniform(-bmax, bmax, size=2) / np.sqrt(2.0)
+    wl = float(H) * rng.uniform(0.8, 2.0)
+    # keep the sinusoid's gradient within max_grad as well
+    amp = min(amp, max_grad * wl / (2 * np.pi))
+    return A, b, amp, wl
+
+
+def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1):
+    """64 independent frame-pair solves of uneven size: one persistent launch
+    (d2ic_cg_solve_batch) vs one launch per problem (d2ic_cg_solve)."""
+    import ctypes
+
+    import torch
+
+    from d2ic_amd import _lib
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([512, 1024, 2048, 4096], size=n, p=[0.4, 0.3, 0.2, 0.1])
+    elems = rng.choice([8, 16, 32], size=n)
+    rois = rng.choice(["rect", "hole"], size=n)
+    t0 = time.perf_counter()
+    probs = []
+    for i in range(n):
+        A, b, amp, wl = _random_motion(rng, int(sizes[i]))
+        probs.append(_problem(rng, int(sizes[i]), int(elems[i]), str(rois[i]), A, b, amp, wl, seed * 1000 + i))
+    log(f"[batch] {n} problems prepared in {time.perf_counter() - t0:.1f} s; "
+        f"{sum(p['np'] for p in probs) / 1e6:.1f} Mpx, {sum(p['nn'] for p in probs)} nodes")
+    st = _lib.stream()
+
+    def run_plain():
+        us, its = [], []
+        for p in probs:
+            u = p["u0"].clone()
+            cnt = torch.zeros(2, dtype=torch.int32, device="cuda")
+            _lib.call("d2ic_cg_solve", p["plan"].handle, _lib.ptr(p["img"]), p["H"], p["H"], _lib.ptr(u), reg,
+                      max_iters, tol, None, _lib.ptr(cnt), ctypes.c_void_p(cnt.data_ptr() + 4), st)
+            us.append(u)
+            its.append(cnt)
+        torch.cuda.synchronize()
+        return us, [int(c[0]) for c in its], [int(c[1]) for c in its]
+
+    def run_batch():
+        us = [p["u0"].clone() for p in probs]
+        cnts = [torch.zeros(2, dtype=torch.int32, device="cuda") for _ in probs]
+        P = len(probs)
+        vp = ctypes.c_void_p
+        arr = lambda xs: (vp * P)(*[vp(x) for x in xs])  # noqa: E731
+        _lib.call("d2ic_cg_solve_batch", P, arr([p["plan"].handle.value for p in probs]),
+                  arr([p["img"].data_ptr() for p in probs]), (ctypes.c_int32 * P)(*[p["H"] for p in probs]),
+                  (ctypes.c_int32 * P)(*[p["H"] for p in probs]), arr([u.data_ptr() for u in us]),
+                  (ctypes.c_float * P)(*([reg] * P)), (ctypes.c_int32 * P)(*([max_iters] * P)),
+                  (ctypes.c_float * P)(*([tol] * P)), None, arr([c.data_ptr() for c in cnts]),
+                  arr([c.data_ptr() + 4 for c in cnts]), st)
+        torch.cuda.synchronize()
+        return us, [int(c[0]) for c in cnts], [int(c[1]) for c in cnts]
+
+    run_batch()  # warm-up (first launch of the batch shape)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ub, itb, pb = run_batch()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    up, itp, pp = run_plain()
+    t_plain = time.perf_counter() - t0
+    same = all(torch.equal(a, b) for a, b in zip(ub, up)) and itb == itp
+    mpx_pass = sum(p["np"] * q for p, q in zip(probs, pb)) / 1e6
+    res = dict(n_problems=n, sizes={int(k): int((sizes == k).sum()) for k in (512, 1024, 2048, 4096)},
+               mpx=sum(p["np"] for p in probs) / 1e6, batched_s=t_batch, plain_s=t_plain,
+               batched_frames_per_s=n / t_batch, plain_frames_per_s=n / t_plain, speedup=t_plain / t_batch,
+               bitwise_equal=bool(same), iterations_mean=float(np.mean(itb)), pixel_passes_total=int(sum(pb)),
+               batched_mpx_pass_per_s=mpx_pass / t_batch, init="ZNCC (win 41, search 4) at <=4096 element centres "
+                                                            "+ IDW, shared by both paths")
+    log(f"[batch] batched {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, speedup {t_plain / t_batch:.2f}, "
+        f"bitwise equal {same}")
+    for p in probs:
+        p["plan"].close()
+    return res
+
+
+def serving_leg(seed=11, n_streams=16, rate=200.0, seconds=1.0, max_iters=200, tol=1e-3, reg=0.1):
+    """16 camera streams, Poisson arrivals (aggregate `rate` frame pairs/s for
+    `seconds`), ROI side log-uniform in [256, 4096], Q4 elements of 8..32 px,
+    smooth affine+sinusoid motion; served (continuous batching) vs plain FIFO."""
+    import ctypes
+
+    import torch
+
+    from d2ic_amd import _lib, speckle
+    from d2ic_amd.mask2mesh import mask_to_mesh
+    from d2ic_amd.mesh_assets import compute_element_centers
+    from d2ic_amd.pixel_assets import build_pixel_assets
+    from d2ic_amd.serving import StreamServer
+
+    rng = np.random.default_rng(seed)
+    # arrivals: one Poisson process per stream
+    arrivals = []
+    for s_ in range(n_streams):
+        t = rng.exponential(n_streams / rate)
+        while t < seconds:
+            arrivals.append((t, s_))
+            t += rng.exponential(n_streams / rate)
+    arrivals.sort()
+    nfr = [sum(1 for _, k in arrivals if k == s_) for s_ in range(n_streams)]
+    t0 = time.perf_counter()
+    streams = []
+    for s_ in range(n_streams):
+        side = int(np.exp(rng.uniform(np.log(256), np.log(4096))))
+        side -= side % 2
+        H = side + 64
+        e = int(rng.integers(8, 33))
+        mask = speckle.rect_mask(H, H, side, side)
+        mesh = mask_to_mesh(mask, e)
+        A, b, amp, wl = _random_motion(rng, side, max_grad=5e-3 / max(nfr[s_], 1), max_amp=3.0)
+        fns = [speckle.affine_field(A * (f + 1), b * (f + 1) / max(nfr[s_], 1), amp * (f + 1) / max(nfr[s_], 1),
+                                    wl, H / 2.0, H / 2.0) for f in range(max(nfr[s_], 1))]
+        ref, frames = speckle.speckle_sequence_device(H, H, seed * 100 + s_, fns, noise_sigma=1.0)
+        pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=mask)
+
+        class _A:
+            element_centers_xy = compute_element_centers(mesh)
+
+        u0 = _zncc_init(mesh, _A, ref, frames[0])
+        streams.append(dict(side=side, e=e, pa=pa, ref=ref, frames=frames, u0=u0))
+    torch.cuda.synchronize()
+    log(f"[serve] {n_streams} streams prepared in {time.perf_counter() - t0:.1f} s; {len(arrivals)} requests; "
+        f"sides {[s['side'] for s in streams]}")
+
+    # served: continuous batching
+    srv = StreamServer(slots=min(64, n_streams), hbm_budget_bytes=200e9, idle_ms=2.0)
+    for k, s_ in enumerate(streams):
+        if not srv.add_stream(k, s_["pa"], s_["ref"], reg_strength=reg, max_iter=max_iters, tol=tol, u0=s_["u0"]):
+            raise RuntimeError("stream does not fit the HBM budget")
+    fidx = [0] * n_streams
+    reqs = []
+    t_start = time.perf_counter()
+    i = 0
+    while i < len(arrivals) or srv.busy():
+        now = time.perf_counter() - t_start
+        while i < len(arrivals) and arrivals[i][0] <= now:
+            k = arrivals[i][1]
+            reqs.append(srv.submit(k, streams[k]["frames"][fidx[k]], t_arrival=t_start + arrivals[i][0]))
+            fidx[k] += 1
+            i += 1
+        srv.pump()
+        if time.perf_counter() - t_start > 300:
+            raise RuntimeError("serving leg timed out")
+    t_end = max(r.t_done for r in reqs)
+    lat_s = np.array([r.t_done - r.t_arrival for r in reqs])
+    served_u = [r.u for r in reqs]
+    srv.close()
+
+    # plain: one solve per launch, first come first served
+    from d2ic_amd.solver_global_cg import CGPlan
+
+    plans = []
+    for s_ in streams:
+        pl = CGPlan(s_["pa"])
+        pl.set_reference(s_["ref"], "cubic")
+        plans.append(pl)
+    uprev = [s_["u0"].clone() for s_ in streams]
+    fidx = [0] * n_streams
+    lat_p, same = [], True
+    st = _lib.stream()
+    torch.cuda.synchronize()
+    t_start_p = time.perf_counter()
+    for j, (ta, k) in enumerate(arrivals):
+        while time.perf_counter() - t_start_p < ta:
+            pass
+        u = uprev[k].clone()
+        cnt = torch.zeros(2, dtype=torch.int32, device="cuda")
+        img = streams[k]["frames"][fidx[k]]
+        _lib.call("d2ic_cg_solve", plans[k].handle, _lib.ptr(img), int(img.shape[0]), int(img.shape[1]),
+                  _lib.ptr(u), reg, max_iters, tol, None, _lib.ptr(cnt), None, st)
+        torch.cuda.synchronize()
+        lat_p.append(time.perf_counter() - (t_start_p + ta))
+        same = same and torch.equal(u, served_u[j])  # requests were served in arrival order too
+        uprev[k] = u
+        fidx[k] += 1
+    t_end_p = time.perf_counter()
+    lat_p = np.array(lat_p)
+    for pl in plans:
+        pl.close()
+    n = len(arrivals)
+    res = dict(streams=n_streams, offered_rate=rate, seconds=seconds, requests=n,
+               sides=[s_["side"] for s_ in streams], elements=[s_["e"] for s_ in streams],
+               served_frames_per_s=n / (t_end - t_start), plain_frames_per_s=n / (t_end_p - t_start_p),
+               served_p50_ms=1e3 * float(np.percentile(lat_s, 50)), served_p99_ms=1e3 * float(np.percentile(lat_s, 99)),
+               plain_p50_ms=1e3 * float(np.percentile(lat_p, 50)), plain_p99_ms=1e3 * float(np.percentile(lat_p, 99)),
+               hbm_budget_bytes=200e9, slots=min(64, n_streams), bitwise_equal=bool(same),
+               init="ZNCC (win 41, search 4) + IDW on each stream's first frame, before the timed run")
+    log(f"[serve] served {res['served_frames_per_s']:.1f} fr/s p50 {res['served_p50_ms']:.1f} ms "
+        f"p99 {res['served_p99_ms']:.1f} ms | plain {res['plain_frames_per_s']:.1f} fr/s "
+        f"p50 {res['plain_p50_ms']:.1f} ms p99 {res['plain_p99_ms']:.1f} ms")
+    return res
+
+
 def measured_traffic():
     """HBM bytes per k_cg_persistent launch on this workload, from the committed
     rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or None."""
@@ -85,6 +330,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-frames", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-streams leg")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -158,6 +406,14 @@ def main() -> int:
     traffic = measured_traffic()
     log(f"[bench] {K} frames in {elapsed:.3f} s -> {fps:.2f} frames/s; iters {iters}; passes {passes}")
 
+    batch = serving = None
+    if not args.no_batch:
+        log("[bench] batched leg: 64 uneven problems")
+        batch = batch_leg()
+    if not args.no_serve:
+        log("[bench] served leg: 16 streams")
+        serving = serving_leg(seconds=args.serve_seconds)
+
     cpu = None
     if not args.no_cpu_baseline and args.cpu_frames > 0:
         log(f"[bench] CPU baseline: oracle on {args.cpu_frames} frame(s)")
@@ -199,6 +455,8 @@ def main() -> int:
             "mean_solve_ms": float(np.mean(solve_ms)),
         },
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     # release device objects before interpreter teardown (profilers finalise then)

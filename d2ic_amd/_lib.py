@@ -49,6 +49,12 @@ _SIGS = {
                           ctypes.POINTER(_i64)],
     "d2ic_serve_poll": [_vp, _i32, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     "d2ic_serve_destroy": [_vp],
+    "d2ic_adj_sizes": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "d2ic_adj_linearize": [_vp, _vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
+    "d2ic_adj_hvp": [_vp, _vp, _f32, _vp, _i32, _vp, _vp, _vp],
+    "d2ic_adj_tangent": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp],
+    "d2ic_adj_image_grads": [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "d2ic_adj_spring": [_vp, _vp, _i32, _vp, _vp, _vp],
     "d2ic_zncc_match": [_vp, _i64, _vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp],
     "d2ic_idw": [_vp, _i64, _vp, _vp, _i64, _vp, _vp],
     "d2ic_strain": [_vp, _vp, _i64, _vp, _vp, _i32, _f32, _f32, _vp, _vp, _vp, _vp],

@@ -25,6 +25,7 @@ SOURCES = {
     "assets.hip": ["-ffp-contract=off"],
     "solver.hip": [],
     "post.hip": [],
+    "adjoint.hip": [],
 }
 
 COMMON = [

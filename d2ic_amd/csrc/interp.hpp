@@ -140,6 +140,80 @@ __device__ __forceinline__ void split_coord(float x, float u, int& x0, float& f)
     f = t - ti;
 }
 
+// Second-derivative weights of the Keys a = -1/2 kernel (taps floor(t)-1 ..
+// floor(t)+2), for the Hessian of the interpolant used by the adjoint.
+__device__ __forceinline__ void keys_weights2(float t, float d2w[4]) {
+    d2w[0] = 0.5f * (-6.0f * t + 4.0f);
+    d2w[1] = 0.5f * (18.0f * t - 10.0f);
+    d2w[2] = 0.5f * (-18.0f * t + 8.0f);
+    d2w[3] = 0.5f * (6.0f * t - 2.0f);
+}
+
+// Taps of either sampler at cell (x0, y0) with fractional offsets (fx, fy):
+// up to 4 x 4 clamped pixel indices with weights w and weight gradients
+// (dw/dx, dw/dy) such that value = sum w * I and grad = sum (dw/dx, dw/dy) * I.
+// Returns the number of taps per axis (4 cubic, 2 linear).
+__device__ __forceinline__ int sampler_taps(int interp, int H, int W, int x0, float fx, int y0, float fy, int col[4],
+                                            int row[4], float wx[4], float dwx[4], float wy[4], float dwy[4]) {
+    if (interp == 0) {
+        keys_weights(fx, wx, dwx);
+        keys_weights(fy, wy, dwy);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            col[j] = clampi(x0 - 1 + j, 0, W - 1);
+            row[j] = clampi(y0 - 1 + j, 0, H - 1);
+        }
+        return 4;
+    }
+    wx[0] = 1.0f - fx;
+    wx[1] = fx;
+    dwx[0] = -1.0f;
+    dwx[1] = 1.0f;
+    wy[0] = 1.0f - fy;
+    wy[1] = fy;
+    dwy[0] = -1.0f;
+    dwy[1] = 1.0f;
+    col[0] = clamp_col(1, x0, H, W);
+    col[1] = clamp_col(1, x0 + 1, H, W);
+    row[0] = clamp_row(1, y0, H, W);
+    row[1] = clamp_row(1, y0 + 1, H, W);
+    return 2;
+}
+
+// Value, gradient and Hessian (hxx, hxy, hyy) of the interpolant.
+__device__ __forceinline__ void sample_hess(int interp, const float* __restrict__ img, int H, int W, int x0, float fx,
+                                            int y0, float fy, float out[6]) {
+    int col[4], row[4];
+    float wx[4], dwx[4], wy[4], dwy[4], d2x[4] = {0.f, 0.f, 0.f, 0.f}, d2y[4] = {0.f, 0.f, 0.f, 0.f};
+    const int n = sampler_taps(interp, H, W, x0, fx, y0, fy, col, row, wx, dwx, wy, dwy);
+    if (n == 4) {
+        keys_weights2(fx, d2x);
+        keys_weights2(fy, d2y);
+    }
+    float v = 0.f, gx = 0.f, gy = 0.f, hxx = 0.f, hxy = 0.f, hyy = 0.f;
+    for (int i = 0; i < n; ++i) {
+        float rv = 0.f, rd = 0.f, rdd = 0.f;
+        for (int j = 0; j < n; ++j) {
+            const float I = img[(size_t)row[i] * W + col[j]];
+            rv += wx[j] * I;
+            rd += dwx[j] * I;
+            rdd += d2x[j] * I;
+        }
+        v += wy[i] * rv;
+        gx += wy[i] * rd;
+        gy += dwy[i] * rv;
+        hxx += wy[i] * rdd;
+        hxy += dwy[i] * rd;
+        hyy += d2y[i] * rv;
+    }
+    out[0] = v;
+    out[1] = gx;
+    out[2] = gy;
+    out[3] = hxx;
+    out[4] = hxy;
+    out[5] = hyy;
+}
+
 // Sample at an absolute coordinate (reference form: floor(x - 0.5)).
 __device__ __forceinline__ void sample(int interp, const float* __restrict__ img, int H, int W, float x, float y,
                                        float& v, float& gx, float& gy) {

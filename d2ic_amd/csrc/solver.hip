@@ -46,10 +46,10 @@
 
 #include "common.hpp"
 #include "interp.hpp"
+#include "plan.hpp"
 
 namespace d2ic {
 
-constexpr int kTile = 256;  // pixels per tile (one element, or a 256-pixel run of one)
 constexpr int kChunks = kTile / 64;  // a wave processes a tile as kChunks chunks of 64 pixels
 constexpr int kBlock = 1024;  // 16 waves: one block per CU
 constexpr int kWavesPerBlock = kBlock / kWave;
@@ -200,10 +200,6 @@ __device__ __forceinline__ int find_prob(const int* pref, int P, int item) {
     return lo;
 }
 
-// Per-tile static record (8 ints, built with the plan): [0] pixel count,
-// [1..4] = floor(min x) - 1, floor(max x), floor(min y) - 1, floor(max y) over
-// the tile's pixels.
-constexpr int kTV = 8;
 constexpr int kLanesPerNode = 8;
 constexpr int kNodesPerWave = kWave / kLanesPerNode;
 constexpr int kNodesPerItem = kNodesPerWave * kWavesPerBlock;  // node item = one block
@@ -1342,35 +1338,6 @@ static int bits_for(int64_t n) {
 }  // namespace d2ic
 
 using namespace d2ic;
-
-struct d2ic_cg_plan {
-    int device = 0;
-    int64_t np = 0, nn = 0, ntiles = 0, nchunks = 0;
-    int nb_maxdeg = 0;
-    int32_t* tile_cnt = nullptr;
-    int4* tile_nodes = nullptr;
-    float* px = nullptr;
-    float* py = nullptr;
-    float4* N4 = nullptr;
-    float* F = nullptr;
-    int32_t* node_ptr = nullptr;
-    int32_t* node_ent = nullptr;
-    int32_t* nb_idx = nullptr;
-    float* nb_wp = nullptr;
-    float* U1 = nullptr;
-    float* G[2] = {nullptr, nullptr};
-    float* D[2] = {nullptr, nullptr};
-    float* TP = nullptr;
-    float* TV = nullptr;
-    double* NCP = nullptr;
-    int ref_H = 0, ref_W = 0, interp = 0;
-    bool has_ref = false;
-    size_t bytes = 0;
-    std::vector<void*> allocs;
-    // start/end of the last persistent-kernel launch this plan led (plans[0])
-    hipEvent_t ev_start = nullptr, ev_end = nullptr;
-    bool timed = false;
-};
 
 // A served session: the mailbox (pinned), the device slot table and the
 // workspace of the persistent kernel, which runs on the session's own stream

@@ -69,6 +69,8 @@ class CGPlan:
         reg_w = dev(pix.node_reg_weight, t.float32)
         self.n_pixels = int(coords.shape[0])
         self.n_nodes = int(nb_deg.shape[0])
+        # nodes that move with the images (pixels or spring neighbours); the others keep u0
+        self.node_active = (dev(pix.node_pixel_degree, t.int32) > 0) | (nb_deg > 0)
         md = int(nb_idx.shape[1]) if nb_idx.dim() == 2 else 0
         h = ctypes.c_void_p(0)
         _lib.call("d2ic_cg_plan_create", _lib.ptr(coords), _lib.ptr(pnodes), _lib.ptr(shapeN), _lib.ptr(pelts),

@@ -112,6 +112,19 @@ def affine_sinusoid(t: float, W: int, H: int, eps: float = 1e-3, amp: float = 1.
     return f
 
 
+def affine_field(A, b, amp: float, wavelength: float, cx: float, cy: float):
+    """u(x, y) = A [x - cx, y - cy] + b, plus amp sin(2 pi x / wavelength) on u_y."""
+    A = np.asarray(A, dtype=np.float64).reshape(2, 2)
+    b = np.asarray(b, dtype=np.float64).reshape(2)
+
+    def f(x, y):
+        dx, dy = x - cx, y - cy
+        ux = A[0, 0] * dx + A[0, 1] * dy + b[0]
+        uy = A[1, 0] * dx + A[1, 1] * dy + b[1] + amp * np.sin(2.0 * np.pi * x / wavelength)
+        return ux, uy
+    return f
+
+
 def plate_with_hole_mask(H: int, W: int, side: int, radius: float) -> np.ndarray:
     """Centred side x side square minus a centred disc (pixel centres tested)."""
     m = np.zeros((H, W), dtype=bool)

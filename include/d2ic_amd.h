@@ -150,6 +150,42 @@ int d2ic_serve_poll(d2ic_server* server, int32_t slot, int64_t ticket, int32_t* 
 int d2ic_serve_destroy(d2ic_server* server);
 
 /* ---------------------------------------------------------------------------
+ * Implicit-function derivatives of a solve (replace jax.grad / jax.jvp
+ * through _cg_solve, solver_global_cg.py:326-469, used for uncertainty
+ * propagation in the reference's notebooks).  At a solution u* of
+ * G(u) = grad_u J = 0: dL/dtheta = -(dG/dtheta)^T lambda with H lambda = dL/du,
+ * and forward tangents du = H^-1 (-(dG/dtheta) dtheta); H = dG/du.
+ * Buffers are device float32 unless noted; sizes from d2ic_adj_sizes.
+ * ------------------------------------------------------------------------ */
+/* lin: 3 floats per pixel slot (r, dg/dx, dg/dy); K: per-tile element
+ * Hessians; tile: per-tile, per-vector work floats. */
+int d2ic_adj_sizes(const d2ic_cg_plan* plan, int64_t* lin_floats, int64_t* k_floats, int64_t* tile_floats);
+
+/* Linearise the image term at u (n_nodes,2) for the deformed image:
+ * gauss_newton = 0 keeps the r * hess(g) term (exact Hessian). */
+int d2ic_adj_linearize(const d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W, const float* u,
+                       int32_t gauss_newton, float* lin, float* K, void* stream);
+
+/* out[k] = H v[k] for nv vectors of (n_nodes,2); work: nv * tile_floats. */
+int d2ic_adj_hvp(const d2ic_cg_plan* plan, const float* K, float alpha_reg, const float* v, int32_t nv, float* out,
+                 float* work, void* stream);
+
+/* b[k] = -(dG/dg) dimg[k] (which = 0, deformed image) or -(dG/df) dimg[k]
+ * (which = 1, reference image) for nv perturbation images (H,W). */
+int d2ic_adj_tangent(const d2ic_cg_plan* plan, int32_t H, int32_t W, const float* u, const float* lin, int32_t which,
+                     const float* dimg, int32_t nv, float* b, float* work, void* stream);
+
+/* grad_f = -(dG/df)^T lambda and grad_g = -(dG/dg)^T lambda as (H,W) images
+ * (either may be NULL); work: H*W 64-bit words when grad_g is wanted.  The
+ * g-image sums are exact fixed-point sums (deterministic). */
+int d2ic_adj_image_grads(const d2ic_cg_plan* plan, int32_t H, int32_t W, const float* u, const float* lin,
+                         const float* lambda, float* grad_f, float* grad_g, unsigned long long* work, void* stream);
+
+/* out[k] = L_W v[k], the spring-energy gradient operator without alpha
+ * (dG/dalpha = L_W u); work: nv * tile_floats. */
+int d2ic_adj_spring(const d2ic_cg_plan* plan, const float* v, int32_t nv, float* out, float* work, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Translation ZNCC (replaces _match_centers_jax, solver_translation_zncc.py:118-168)
  * ------------------------------------------------------------------------ */
 /* centers (n,2) f32; ref (Hr,Wr) and def (Hd,Wd) f32 row-major.  u_out (n,2)
