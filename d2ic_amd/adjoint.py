@@ -1,0 +1,232 @@
+"""Differentiation of global solves through hand-written adjoint kernels.
+
+The reference differentiates through ``_cg_solve`` with ``jax.grad`` /
+``jax.jvp`` (uncertainty propagation; reference notebook 11).  Here the
+solution u* of G(u) = grad_u J(u; f, g, alpha) = 0 (``solver_global_cg.py:
+289-323``) is differentiated implicitly:
+
+* reverse mode: dL/dtheta = -(dG/dtheta)^T lambda with H lambda = dL/du
+  (one linear CG solve with Hessian-vector products, ``d2ic_adj_hvp``);
+* forward mode: du = H^-1 (-(dG/dtheta) dtheta) for perturbation images
+  (``d2ic_adj_tangent``), batched over many probes.
+
+H is the exact Hessian of J at u* (including the r * hess(g) term) unless
+``gauss_newton=True``.  Nodes that no pixel and no spring touches keep u0 in
+the solver; their derivatives are zero and they are masked out of the linear
+solves.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .solver_global_cg import CGPlan
+
+
+class Linearization:
+    """The image term linearised at u for one deformed image."""
+
+    def __init__(self, plan: CGPlan, def_image, u, gauss_newton: bool = False):
+        t = _lib.require_gpu()
+        self.plan = plan
+        self.g = _lib.to_dev(def_image, t.float32)
+        self.H, self.W = int(self.g.shape[0]), int(self.g.shape[1])
+        self.u = _lib.to_dev(u, t.float32, (plan.n_nodes, 2)).contiguous()
+        a, b, c = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.call("d2ic_adj_sizes", plan.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self.tile_floats = int(c.value)
+        dev = _lib.device()
+        self.lin = t.empty(max(int(a.value), 1), dtype=t.float32, device=dev)
+        self.K = t.empty(max(int(b.value), 1), dtype=t.float32, device=dev)
+        self.mask = plan.node_active.to(t.float32)[:, None]
+        _lib.call("d2ic_adj_linearize", plan.handle, _lib.ptr(self.g), self.H, self.W, _lib.ptr(self.u),
+                  int(bool(gauss_newton)), _lib.ptr(self.lin), _lib.ptr(self.K), _lib.stream())
+
+    def _work(self, nv):
+        t = _lib.torch()
+        return t.empty(max(nv * self.tile_floats, 1), dtype=t.float32, device=_lib.device())
+
+    def hvp(self, V, alpha):
+        """H v for V of shape (nv, n_nodes, 2)."""
+        t = _lib.torch()
+        V = V.contiguous()
+        out = t.empty_like(V)
+        nv = int(V.shape[0])
+        _lib.call("d2ic_adj_hvp", self.plan.handle, _lib.ptr(self.K), float(alpha), _lib.ptr(V), nv,
+                  _lib.ptr(out), _lib.ptr(self._work(nv)), _lib.stream())
+        return out
+
+    def spring(self, V):
+        """L_W v (spring-energy gradient operator, without alpha)."""
+        t = _lib.torch()
+        V = V.contiguous()
+        out = t.empty_like(V)
+        nv = int(V.shape[0])
+        _lib.call("d2ic_adj_spring", self.plan.handle, _lib.ptr(V), nv, _lib.ptr(out), _lib.ptr(self._work(nv)),
+                  _lib.stream())
+        return out
+
+    def solve(self, B, alpha, rtol: float = 1e-6, max_iter: int = 2000, check_every: int = 10):
+        """Conjugate gradients on H X = B for every right-hand side of B (nv, n_nodes, 2);
+        FP64 dot products, per-RHS step sizes."""
+        t = _lib.torch()
+        m = self.mask
+        B = B * m
+        X = t.zeros_like(B)
+        R = B.clone()
+        P = R.clone()
+        rr = (R.double() ** 2).sum(dim=(1, 2))
+        stop = (rtol ** 2) * rr.clamp_min(1e-300)
+        for it in range(max_iter):
+            HP = self.hvp(P, alpha) * m
+            php = (P.double() * HP.double()).sum(dim=(1, 2))
+            a = t.where(php > 0, rr / php.clamp_min(1e-300), t.zeros_like(rr))
+            X = X + a.float()[:, None, None] * P
+            R = R - a.float()[:, None, None] * HP
+            rr_new = (R.double() ** 2).sum(dim=(1, 2))
+            if (it + 1) % check_every == 0 and bool((rr_new <= stop).all()):
+                break
+            beta = t.where(rr > 0, rr_new / rr.clamp_min(1e-300), t.zeros_like(rr))
+            P = R + beta.float()[:, None, None] * P
+            rr = rr_new
+        return X
+
+    def tangent_rhs(self, dimg, which: str = "g"):
+        """-(dG/dg) dg (which='g') or -(dG/df) df (which='f') for images (nv, H, W)."""
+        t = _lib.torch()
+        D = _lib.to_dev(dimg, t.float32)
+        if D.dim() == 2:
+            D = D[None]
+        nv = int(D.shape[0])
+        B = t.empty((nv, self.plan.n_nodes, 2), dtype=t.float32, device=_lib.device())
+        _lib.call("d2ic_adj_tangent", self.plan.handle, self.H, self.W, _lib.ptr(self.u), _lib.ptr(self.lin),
+                  0 if which == "g" else 1, _lib.ptr(D.contiguous()), nv, _lib.ptr(B), _lib.ptr(self._work(nv)),
+                  _lib.stream())
+        return B
+
+    def tangent(self, dimg, alpha, which: str = "g", **cg):
+        """Forward-mode derivative du of the solution for perturbation images (nv, H, W)."""
+        return self.solve(self.tangent_rhs(dimg, which), alpha, **cg)
+
+    def image_grads(self, lam, want_f: bool = True, want_g: bool = True):
+        """(dL/df, dL/dg) images for the adjoint state lambda (n_nodes, 2)."""
+        t = _lib.torch()
+        lam = _lib.to_dev(lam, t.float32, (self.plan.n_nodes, 2)).contiguous()
+        dev = _lib.device()
+        gf = t.empty((self.H, self.W), dtype=t.float32, device=dev) if want_f else None
+        gg = t.empty((self.H, self.W), dtype=t.float32, device=dev) if want_g else None
+        work = t.empty((self.H * self.W,), dtype=t.int64, device=dev) if want_g else None
+        _lib.call("d2ic_adj_image_grads", self.plan.handle, self.H, self.W, _lib.ptr(self.u), _lib.ptr(self.lin),
+                  _lib.ptr(lam), _lib.ptr(gf), _lib.ptr(gg), _lib.ptr(work), _lib.stream())
+        return gf, gg
+
+
+def _solve(plan: CGPlan, def_image, u0, alpha, max_iter, tol):
+    t = _lib.torch()
+    g = _lib.to_dev(def_image, t.float32)
+    u = _lib.to_dev(u0, t.float32, (plan.n_nodes, 2)).clone()
+    cnt = t.zeros(2, dtype=t.int32, device=_lib.device())
+    _lib.call("d2ic_cg_solve", plan.handle, _lib.ptr(g), int(g.shape[0]), int(g.shape[1]), _lib.ptr(u), float(alpha),
+              int(max_iter), float(tol), None, _lib.ptr(cnt), ctypes.c_void_p(cnt.data_ptr() + 4), _lib.stream())
+    return u, int(cnt[0].item())
+
+
+def _make_fn():
+    torch = _lib.torch()
+
+    class ImplicitGlobalSolve(torch.autograd.Function):
+        """u* = argmin_u J(u; f, g, alpha) with implicit-function gradients."""
+
+        @staticmethod
+        def forward(ctx, ref_image, def_image, reg_strength, plan, u0, max_iter, tol, interpolation, cg_opts):
+            plan.set_reference(ref_image.detach(), interpolation)
+            alpha = float(reg_strength.detach().item())
+            u, n = _solve(plan, def_image.detach(), u0, alpha, max_iter, tol)
+            ctx.plan, ctx.alpha, ctx.cg_opts = plan, alpha, cg_opts
+            ctx.save_for_backward(def_image.detach(), u)
+            ctx.n_iters = n
+            return u
+
+        @staticmethod
+        def backward(ctx, grad_u):
+            g, u = ctx.saved_tensors
+            lin = Linearization(ctx.plan, g, u, gauss_newton=ctx.cg_opts.get("gauss_newton", False))
+            opts = {k: v for k, v in ctx.cg_opts.items() if k != "gauss_newton"}
+            lam = lin.solve(grad_u.float().contiguous()[None], ctx.alpha, **opts)[0]
+            gf, gg = lin.image_grads(lam, ctx.needs_input_grad[0], ctx.needs_input_grad[1])
+            galpha = None
+            if ctx.needs_input_grad[2]:
+                galpha = -(lam.double() * lin.spring(u[None])[0].double()).sum().to(torch.float32)
+            return gf, gg, galpha, None, None, None, None, None, None
+
+    return ImplicitGlobalSolve
+
+
+_FN = None
+
+
+def solve_differentiable(plan: CGPlan, ref_image, def_image, reg_strength, u0=None, *, max_iter: int = 200,
+                         tol: float = 1e-3, interpolation: str = "cubic", gauss_newton: bool = False,
+                         cg_rtol: float = 1e-6, cg_max_iter: int = 2000):
+    """Global solve whose output carries gradients w.r.t. both images and
+    reg_strength (torch tensors with requires_grad)."""
+    global _FN
+    if _FN is None:
+        _FN = _make_fn()
+    t = _lib.torch()
+    if not isinstance(reg_strength, t.Tensor):
+        reg_strength = t.tensor(float(reg_strength), device=_lib.device())
+    if u0 is None:
+        u0 = t.zeros((plan.n_nodes, 2), dtype=t.float32, device=_lib.device())
+    opts = dict(gauss_newton=gauss_newton, rtol=cg_rtol, max_iter=cg_max_iter)
+    return _FN.apply(ref_image, def_image, reg_strength, plan, u0, max_iter, tol, interpolation, opts)
+
+
+def green_lagrange_torch(u, nodes, nb_index, nb_degree, gauge_length: float = 0.0, eps: float = 1e-8):
+    """Nodal LSQ Green-Lagrange tensor (``strain.py:11-114``) as differentiable
+    torch code: returns E (n_nodes, 2, 2)."""
+    t = _lib.torch()
+    X = _lib.to_dev(nodes, t.float32).reshape(-1, 2).to(u.dtype)
+    idx = _lib.to_dev(nb_index, t.int64)
+    deg = _lib.to_dev(nb_degree, t.int64).reshape(-1)
+    md = idx.shape[1]
+    valid = (t.arange(md, device=idx.device)[None, :] < deg[:, None])
+    dX = X[idx] - X[:, None, :]
+    dU = u[idx] - u[:, None, :]
+    if gauge_length > 0:
+        w = t.exp(-((dX ** 2).sum(-1)) / (gauge_length ** 2))
+    else:
+        w = t.ones(dX.shape[:2], dtype=u.dtype, device=u.device)
+    w = w * valid.to(u.dtype)
+    A = t.einsum("nk,nki,nkj->nij", w, dX, dX) + eps * t.eye(2, dtype=u.dtype, device=u.device)
+    Bm = t.einsum("nk,nki,nkj->nij", w, dU, dX)  # rows: component of u, cols: direction
+    grad_u = t.linalg.solve(A, Bm.transpose(1, 2)).transpose(1, 2)
+    F = t.eye(2, dtype=u.dtype, device=u.device) + grad_u
+    return 0.5 * (F.transpose(1, 2) @ F - t.eye(2, dtype=u.dtype, device=u.device))
+
+
+def strain_noise_std(lin: Linearization, alpha, n_probes: int = 256, sigma: float = 1.0, seed: int = 0,
+                     batch: int = 64, strain_fn=None, **cg):
+    """Noise-to-displacement (and optionally strain) standard deviations from
+    forward-tangent probes with i.i.d. N(0, sigma^2) noise on the deformed image."""
+    t = _lib.torch()
+    gen = t.Generator(device=_lib.device())
+    gen.manual_seed(seed)
+    du_sq = t.zeros((lin.plan.n_nodes, 2), dtype=t.float64, device=_lib.device())
+    e_sq = None
+    done = 0
+    while done < n_probes:
+        nv = min(batch, n_probes - done)
+        noise = sigma * t.randn((nv, lin.H, lin.W), generator=gen, device=_lib.device(), dtype=t.float32)
+        du = lin.tangent(noise, alpha, "g", **cg)
+        du_sq += (du.double() ** 2).sum(0)
+        if strain_fn is not None:
+            e = t.stack([strain_fn(du[k]) for k in range(nv)]).double()
+            e_sq = (e ** 2).sum(0) if e_sq is None else e_sq + (e ** 2).sum(0)
+        done += nv
+    out = dict(u_std=t.sqrt(du_sq / n_probes).float())
+    if e_sq is not None:
+        out["strain_std"] = t.sqrt(e_sq / n_probes).float()
+    return out
