@@ -290,6 +290,80 @@ def serving_leg(seed=11, n_streams=16, rate=200.0, seconds=1.0, max_iters=200, t
     return res
 
 
+def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
+    """1024x1024 speckle pair, Q4-16 on an 800x800 ROI, config-1 style field
+    (frame 10): forward solve, loss = |u - u_target|^2 + mean E_xx over a
+    200x200 window, gradients w.r.t. both images and reg_strength through the
+    implicit adjoint, and 256 forward-tangent noise probes."""
+    import torch
+
+    from d2ic_amd import speckle
+    from d2ic_amd.adjoint import Linearization, green_lagrange_torch, solve_differentiable
+    from d2ic_amd.mask2mesh import mask_to_mesh
+    from d2ic_amd.mesh_assets import build_node_neighbor_tables
+    from d2ic_amd.pixel_assets import build_pixel_assets
+    from d2ic_amd.solver_global_cg import CGPlan
+
+    H = W = 1024
+    mask = speckle.rect_mask(H, W, 800, 800)
+    mesh = mask_to_mesh(mask, 16)
+    fn = speckle.affine_sinusoid(10, W, H)
+    ref, frames = speckle.speckle_sequence_device(H, W, seed, [fn], noise_sigma=1.0)
+    g = frames[0]
+    pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=mask)
+    plan = CGPlan(pa)
+    nb_idx, nb_deg = build_node_neighbor_tables(mesh)
+    X = np.asarray(mesh.nodes_xy, np.float64)
+    ux, uy = fn(X[:, 0], X[:, 1])
+    act = plan.node_active
+    win = torch.tensor((np.abs(X[:, 0] - W / 2) <= 100) & (np.abs(X[:, 1] - H / 2) <= 100), device="cuda") & act
+    u_target = torch.tensor(np.stack([ux, uy], 1), dtype=torch.float32, device="cuda")
+    f = ref.clone().requires_grad_(True)
+    gg = g.clone().requires_grad_(True)
+    alpha = torch.tensor(reg, device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u = solve_differentiable(plan, f, gg, alpha, max_iter=max_iters, tol=tol, cg_rtol=1e-6)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    E = green_lagrange_torch(u, X, nb_idx, nb_deg)
+    loss = ((u - u_target)[act] ** 2).sum() + E[win, 0, 0].mean()
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    lin = Linearization(plan, g, u.detach())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    du_sq = torch.zeros_like(u, dtype=torch.float64)
+    exx_sq = torch.zeros(plan.n_nodes, dtype=torch.float64, device="cuda")
+    for k0 in range(0, n_probes, 64):
+        nv = min(64, n_probes - k0)
+        noise = torch.randn((nv, H, W), generator=gen, device="cuda")
+        du = lin.tangent(noise, reg, "g", rtol=1e-4)
+        du_sq += (du.double() ** 2).sum(0)
+        # first-order E_xx change along each probe (central difference around u*)
+        for k in range(nv):
+            e1 = green_lagrange_torch(u.detach() + 1e-3 * du[k], X, nb_idx, nb_deg)[:, 0, 0]
+            e0 = green_lagrange_torch(u.detach() - 1e-3 * du[k], X, nb_idx, nb_deg)[:, 0, 0]
+            exx_sq += (((e1 - e0) / 2e-3).double() ** 2)
+    torch.cuda.synchronize()
+    t_probe = time.perf_counter() - t0
+    u_std = torch.sqrt(du_sq / n_probes)[act]
+    exx_std = torch.sqrt(exx_sq / n_probes)[win]
+    res = dict(image=[H, W], n_pixels=plan.n_pixels, n_nodes=plan.n_nodes, loss=float(loss),
+               forward_solve_s=t_fwd, backward_s=t_bwd, probes=n_probes, probes_s=t_probe,
+               grad_norms=dict(ref=float(f.grad.norm()), deformed=float(gg.grad.norm()), reg=float(alpha.grad)),
+               u_std_median_px=float(u_std.median()), exx_std_window_median=float(exx_std.median()),
+               check="finite differences of FP64 oracle solves in tests/test_gpu_adjoint.py (rel. err ~2e-4)")
+    log(f"[diff] forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms, {n_probes} probes {t_probe:.2f} s; "
+        f"u_std median {res['u_std_median_px']:.2e} px, exx_std median {res['exx_std_window_median']:.2e}")
+    plan.close()
+    return res
+
+
 def measured_traffic():
     """HBM bytes per k_cg_persistent launch on this workload, from the committed
     rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or None."""
@@ -333,6 +407,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-streams leg")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -414,6 +489,11 @@ def main() -> int:
         log("[bench] served leg: 16 streams")
         serving = serving_leg(seconds=args.serve_seconds)
 
+    diff = None
+    if not args.no_diff:
+        log("[bench] differentiation leg: 1024x1024")
+        diff = differentiation_leg()
+
     cpu = None
     if not args.no_cpu_baseline and args.cpu_frames > 0:
         log(f"[bench] CPU baseline: oracle on {args.cpu_frames} frame(s)")
@@ -457,6 +537,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serving,
+        "differentiation": diff,
     }
     print(json.dumps(line), flush=True)
     # release device objects before interpreter teardown (profilers finalise then)
