@@ -486,8 +486,10 @@ def main() -> int:
         log("[bench] batched leg: 64 uneven problems")
         batch = batch_leg()
     if not args.no_serve:
-        log("[bench] served leg: 16 streams")
-        serving = serving_leg(seconds=args.serve_seconds)
+        log("[bench] served leg: 16 streams, offered 200 frame pairs/s")
+        serving = {"offered_200_per_s": serving_leg(rate=200.0, seconds=args.serve_seconds)}
+        log("[bench] served leg: 16 streams, offered 50 frame pairs/s")
+        serving["offered_50_per_s"] = serving_leg(rate=50.0, seconds=2.0 * args.serve_seconds)
 
     diff = None
     if not args.no_diff:

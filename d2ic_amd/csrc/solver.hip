@@ -815,6 +815,22 @@ __device__ void serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
     __threadfence();
 }
 
+// Served mode, every block after the barrier: copy the descriptors of the
+// slots admitted at this barrier into the block's LDS table.
+__device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, Workspace* ws) {
+    constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
+    const unsigned long long admit = __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (admit) {
+        for (int i = threadIdx.x; i < kMaxSlots * kWords; i += kBlock) {
+            const int slot = i / kWords;
+            if ((admit >> slot) & 1ull)
+                reinterpret_cast<unsigned long long*>(s_desc)[i] = __hip_atomic_load(
+                    reinterpret_cast<const unsigned long long*>(gdesc) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+}
+
 // wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
 // a wait is abandoned only after both 2^20 polls and 4 s.
 constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
@@ -831,8 +847,10 @@ template <bool kServe>
 using DescTable = typename std::conditional<kServe, ProbDesc*, const ProbDesc* __restrict__>::type;
 
 template <bool kServe>
-__global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> descs, int P, ProbState* __restrict__ S,
+__global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdesc, int P, ProbState* __restrict__ S,
                                                           Workspace* __restrict__ ws, int max_steps, ServeHost* sh) {
+    // served sessions read descriptors from an LDS copy refreshed on admission
+    __shared__ ProbDesc s_desc[kServe ? kMaxSlots : 1];
     __shared__ int s_flags[kMaxProblems];
     __shared__ float s_alpha[kMaxProblems];
     __shared__ float s_beta[kMaxProblems];
@@ -860,6 +878,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> desc
     const int gwave = __builtin_amdgcn_readfirstlane(vb * kWavesPerBlock + wave);
     const int nwaves = nb * kWavesPerBlock;
 
+    const ProbDesc* descs = gdesc;
+    if constexpr (kServe) {
+        constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
+        for (int i = threadIdx.x; i < P * kWords; i += kBlock)
+            reinterpret_cast<unsigned long long*>(s_desc)[i] = reinterpret_cast<const unsigned long long*>(gdesc)[i];
+        descs = s_desc;
+    }
     // first step's plan (from k_init) and the problem states
     for (int p = threadIdx.x; p < P; p += kBlock) {
         const PlanEntry e = ws->plan[p];
@@ -1054,7 +1079,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> desc
                 }
                 __threadfence();
                 if constexpr (kServe)
-                    serve_admit(descs, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
+                    serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
                 for (int b = lane; b < nb; b += kWave)
                     __hip_atomic_store(&ws->release[b * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -1084,6 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> desc
         __syncthreads();
         if (!s_ok) break;
         if (sh && __hip_atomic_load(&ws->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if constexpr (kServe) serve_refresh(gdesc, s_desc, ws);
         block_decide(descs, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_fin, s_sr2,
                      s_badp);
     }
