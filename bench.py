@@ -353,7 +353,7 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
     t_probe = time.perf_counter() - t0
     u_std = torch.sqrt(du_sq / n_probes)[act]
     exx_std = torch.sqrt(exx_sq / n_probes)[win]
-    res = dict(image=[H, W], n_pixels=plan.n_pixels, n_nodes=plan.n_nodes, loss=float(loss),
+    res = dict(image=[H, W], n_pixels=plan.n_pixels, n_nodes=plan.n_nodes, loss=float(loss.detach()),
                forward_solve_s=t_fwd, backward_s=t_bwd, probes=n_probes, probes_s=t_probe,
                grad_norms=dict(ref=float(f.grad.norm()), deformed=float(gg.grad.norm()), reg=float(alpha.grad)),
                u_std_median_px=float(u_std.median()), exx_std_window_median=float(exx_std.median()),

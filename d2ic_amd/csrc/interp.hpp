@@ -38,15 +38,18 @@ __device__ __forceinline__ void sample_linear_at(const float* __restrict__ img_,
 // weights of taps floor(t)-1 .. floor(t)+2 for fractional offset t and their
 // t-derivatives.  Boundary "nearest": tap indices clamp to the image.
 __device__ __forceinline__ void keys_weights(float t, float w[4], float dw[4]) {
-    const float t2 = t * t, t3 = t2 * t;
-    w[0] = 0.5f * (-t3 + 2.0f * t2 - t);
-    w[1] = 0.5f * (3.0f * t3 - 5.0f * t2 + 2.0f);
-    w[2] = 0.5f * (-3.0f * t3 + 4.0f * t2 + t);
-    w[3] = 0.5f * (t3 - t2);
-    dw[0] = 0.5f * (-3.0f * t2 + 4.0f * t - 1.0f);
-    dw[1] = 0.5f * (9.0f * t2 - 10.0f * t);
-    dw[2] = 0.5f * (-9.0f * t2 + 8.0f * t + 1.0f);
-    dw[3] = 0.5f * (3.0f * t2 - 2.0f * t);
+    // Horner forms of w0 = (-t^3 + 2t^2 - t)/2, w1 = (3t^3 - 5t^2 + 2)/2,
+    // w2 = (-3t^3 + 4t^2 + t)/2, w3 = (t^3 - t^2)/2 and their derivatives
+    const float t2 = t * t, h = 0.5f * t;
+    const float p2 = __fmaf_rn(-1.5f, t, 2.0f);  // shared by w2 and dw0
+    w[0] = __fmaf_rn(__fmaf_rn(-0.5f, t, 1.0f), t, -0.5f) * t;
+    w[1] = __fmaf_rn(__fmaf_rn(1.5f, t, -2.5f), t2, 1.0f);
+    w[2] = __fmaf_rn(p2, t2, h);
+    w[3] = __fmaf_rn(0.5f, t, -0.5f) * t2;
+    dw[0] = __fmaf_rn(p2, t, -0.5f);
+    dw[1] = __fmaf_rn(4.5f, t, -5.0f) * t;
+    dw[2] = __fmaf_rn(__fmaf_rn(-4.5f, t, 4.0f), t, 0.5f);
+    dw[3] = __fmaf_rn(1.5f, t, -1.0f) * t;
 }
 
 __device__ __forceinline__ void sample_cubic_at(const float* __restrict__ img_, int H, int W, int x0, float tx, int y0,

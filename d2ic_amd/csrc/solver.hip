@@ -167,6 +167,23 @@ __device__ __forceinline__ void st2(float* p, int n, float2 v) {
     gst(reinterpret_cast<f2v*>(p))[n] = w;
 }
 
+// Stores of data that other blocks read after the next grid barrier go
+// through to memory (agent-scope relaxed atomic stores): the L2 then holds no
+// dirty lines for them, so the barrier's release fence has little to write
+// back (the fences themselves are unchanged).
+__device__ __forceinline__ void st_wt(D2IC_GLOBAL float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_wt(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st2_wt(float* p, int n, float2 v) {
+    const unsigned long long bits =
+        (unsigned long long)__float_as_uint(v.x) | ((unsigned long long)__float_as_uint(v.y) << 32);
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p) + n, bits, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Search direction d = -g (restart) or -g + beta * d_prev, single rounding per
 // component so every place that forms it gets the same bits.
 __device__ __forceinline__ float2 direction(float2 g, float2 dp, float beta, bool restart) {
@@ -238,10 +255,12 @@ __device__ __forceinline__ float tile_vals(const ProbDesc* __restrict__ D, int i
 
 __device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int t, int c, int lane) {
     TilePix r;
-    const size_t i = (size_t)t * kTile + c * kWave + lane;
-    r.x = gld(D->px)[i];
-    r.y = gld(D->py)[i];
-    const f4v nv = gld(reinterpret_cast<const f4v*>(D->N4))[i];
+    // wave-uniform tile base + 32-bit lane offset (scalar base addressing)
+    const size_t t0 = (size_t)t * kTile;
+    const int i = c * kWave + lane;
+    r.x = gld(D->px + t0)[i];
+    r.y = gld(D->py + t0)[i];
+    const f4v nv = gld(reinterpret_cast<const f4v*>(D->N4 + t0))[i];
     r.N = make_float4(nv.x, nv.y, nv.z, nv.w);
     return r;
 }
@@ -314,6 +333,29 @@ __device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const T
     const int n = g.bw * g.bh, interp = D->interp, H = D->H, W = D->W;
     const D2IC_GLOBAL float* img = gld(D->img);
     const float inv_bw = 1.0f / (float)g.bw;
+    // the bilinear sampler clips by the transposed shape: its taps are the
+    // pixels themselves only inside [0, min(H, W))^2
+    const int lim_w = interp == 0 ? W : min(W, H), lim_h = interp == 0 ? H : min(W, H);
+    if (g.bx0 >= 0 && g.by0 >= 0 && g.bx0 + g.bw <= lim_w && g.by0 + g.bh <= lim_h) {
+        // box inside the image: no clamps; element k = j * 64 + lane at (r, c),
+        // advanced by 64 = rs * bw + cs per register; rows past the box repeat
+        // its last row (those values are never stored)
+        const D2IC_GLOBAL float* base = img + (size_t)g.by0 * W + g.bx0;
+        const int rs = (int)(64.5f * inv_bw), cs = kWave - rs * g.bw;
+        int r = (int)(((float)lane + 0.5f) * inv_bw);
+        int c = lane - r * g.bw;
+#pragma unroll
+        for (int j = 0; j < kBoxRegs; ++j) {
+            if (j * kWave < n) v[j] = base[min(r, g.bh - 1) * W + c];
+            c += cs;
+            r += rs;
+            if (c >= g.bw) {
+                c -= g.bw;
+                r += 1;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < kBoxRegs; ++j) {
         if (j * kWave < n) {
@@ -343,11 +385,11 @@ __device__ __forceinline__ void box_store(const TileBox& g, int lane, const floa
 // accumulated per lane into r^2 (FP64) and the 8 gradient partials
 // acc[2a + {0,1}] += r * (dg/dx, dg/dy) * N_a.  Samples inside the staged box
 // read LDS; others read global memory (same values).  Dead lanes add zeros.
-__device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, size_t i, const TilePix& px,
+__device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, const float* Ft, int i, const TilePix& px,
                                             const float2 (&uc)[4], const TileBox& g, bool live,
                                             const __attribute__((address_space(3))) float* box, float (&acc)[8],
                                             double& r2) {
-    const float f = gld(D->F)[i];
+    const float f = gld(Ft)[i];
     const float4 N = px.N;
     const float ux = N.x * uc[0].x + N.y * uc[1].x + N.z * uc[2].x + N.w * uc[3].x;
     const float uy = N.x * uc[0].y + N.y * uc[1].y + N.z * uc[2].y + N.w * uc[3].y;
@@ -405,7 +447,7 @@ __device__ __forceinline__ void reduce8_store(const float (&v)[8], int lane, D2I
     y += dpp_f<0x128>(y);
     y += __shfl_xor(y, 16, kWave);
     y += __shfl_xor(y, 32, kWave);
-    if (lane < 8) out[(b2 ? 4 : 0) + (b0 ? 2 : 0) + (b1 ? 1 : 0)] = y;
+    if (lane < 8) st_wt(out + ((b2 ? 4 : 0) + (b0 ? 2 : 0) + (b1 ? 1 : 0)), y);
 }
 
 // sum over the 8 lanes of an aligned lane group (DPP quad_perm and
@@ -439,8 +481,8 @@ __device__ __forceinline__ void block_store2(double a, double b, double* s_red, 
             x += s_red[2 * w];
             y += s_red[2 * w + 1];
         }
-        gst(out)[0] = x;
-        gst(out)[1] = y;
+        st_wt(out, x);
+        st_wt(out + 1, y);
     }
     __syncthreads();
 }
@@ -463,7 +505,7 @@ __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags,
         const float2 g = init ? make_float2(0.f, 0.f) : ld2(G, n);
         const float2 d = init ? make_float2(0.f, 0.f) : direction(g, ld2(Dp, n), beta, restart);
         if (sub == 0) {
-            st2(D->D[dcur ^ 1], n, d);
+            st2_wt(D->D[dcur ^ 1], n, d);
             gd = (double)g.x * (double)d.x + (double)g.y * (double)d.y;
         }
         const float2 ucn = step_to(ld2(U, n), d, alpha);
@@ -536,8 +578,8 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
         const float gx = (float)(sx * D->invNp) + D->alpha_reg * rx;
         const float gy = (float)(sy * D->invNp) + D->alpha_reg * ry;
         if (sub == 0) {
-            st2(D->U[cur ^ 1], n, un);
-            st2(D->G[cur ^ 1], n, make_float2(gx, gy));
+            st2_wt(D->U[cur ^ 1], n, un);
+            st2_wt(D->G[cur ^ 1], n, make_float2(gx, gy));
             gg = (double)gx * (double)gx + (double)gy * (double)gy;
             gnum = (double)gx * ((double)gx - (double)go.x) + (double)gy * ((double)gy - (double)go.y);
         }
@@ -998,7 +1040,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                             pn = tile_pix(Da, ta, c + 1, lane);
                         else if (it + 1 < t1)
                             pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
-                        chunk_accum(Da, (size_t)ta * kTile + c * kWave + lane, px, uc, b_cur,
+                        chunk_accum(Da, Da->F + (size_t)ta * kTile, c * kWave + lane, px, uc, b_cur,
                                     c * kWave + lane < b_cur.cnt, my_box, acc, r2);
                         px = pn;
                     }
