@@ -110,6 +110,23 @@ __device__ __forceinline__ float wave_sum63(float v) {
     return v;
 }
 
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+    const unsigned lo = (unsigned)dpp_i<Ctrl, RowMask>((int)(unsigned)v);
+    const unsigned hi = (unsigned)dpp_i<Ctrl, RowMask>((int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum63(unsigned long long v) {
+    v += dpp_u64<0xB1>(v);
+    v += dpp_u64<0x4E>(v);
+    v += dpp_u64<0x141>(v);
+    v += dpp_u64<0x140>(v);
+    v += dpp_u64<0x142, 0xA>(v);
+    v += dpp_u64<0x143, 0xC>(v);
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum63(double v) {
     v += dpp_d<0xB1>(v);
     v += dpp_d<0x4E>(v);

@@ -106,6 +106,32 @@ struct ProbState {
     double J, gg;
 };
 
+// Per-problem descriptor fields read by the step decisions, kept in LDS so
+// that the serial decision code never waits on global memory.
+struct DecideDesc {
+    double invNp;
+    const double* NCP;
+    float* hist;
+    int32_t* n_iters;
+    int32_t* pass_count;
+    float alpha_reg, tol;
+    int max_iter, ntiles, nchunks, pad;
+};
+
+__device__ __forceinline__ void fill_decide_desc(DecideDesc& o, const ProbDesc& d) {
+    o.invNp = d.invNp;
+    o.NCP = d.NCP;
+    o.hist = d.hist;
+    o.n_iters = d.n_iters;
+    o.pass_count = d.pass_count;
+    o.alpha_reg = d.alpha_reg;
+    o.tol = d.tol;
+    o.max_iter = d.max_iter;
+    o.ntiles = d.ntiles;
+    o.nchunks = d.nchunks;
+    o.pad = 0;
+}
+
 // What every block needs for the next step.
 struct PlanEntry {
     int flags;
@@ -155,6 +181,7 @@ struct Workspace {
 
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float2 ld2(const float* p, int n) {
     const f2v v = gld(reinterpret_cast<const f2v*>(p))[n];
@@ -590,7 +617,7 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
 // State transition of one problem after a step (root block only).  just_did_a:
 // the finished step was an A-step (Armijo decision follows), else a B-step
 // (stop test and beta follow).  fin: node-chunk sums of the finished step.
-__device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double sr2, bool bad, double fin0,
+__device__ int decide(const DecideDesc& D, ProbState& st, bool just_did_a, double sr2, bool bad, double fin0,
                       double fin1, bool writer) {
     int flags = 0;
     if (st.mode == MODE_DONE) return 0;
@@ -677,101 +704,111 @@ __device__ int decide(const ProbDesc& D, ProbState& st, bool just_did_a, double 
 // Every block, right after barrier `step`: reduce the finished step from the
 // same memory in the same order, run the same state machine on its own copy
 // of the problem states, and derive the next step's plan (identical in every
-// block).  Block 0 alone writes histories, n_iters and the global states, and
-// clears the accumulators of step + 2.
-__device__ void block_decide(const ProbDesc* descs, int P, ProbState* __restrict__ S, Workspace* ws, ServeHost* sh,
+// block).  One wave per problem: all loads of the step's sums are issued
+// before the first use and reduced through DPP; the wave's last lane runs the
+// state machine.  Block 0 alone writes histories, n_iters and the global
+// states, and clears the accumulators of step + 2.
+__device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restrict__ S, Workspace* ws, ServeHost* sh,
                              int step, ProbState* s_st, int* s_flags, float* s_alpha, float* s_beta, int* s_toff,
-                             int* s_coff, int* s_hdr, double (*s_fin)[2], double* s_sr2, int* s_badp) {
+                             int* s_coff, int* s_hdr, unsigned long long* s_hi, unsigned long long* s_lo,
+                             int* s_bad) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const bool just_did_a = (step & 1) == 0;
     const int par = step & 1, buf = step % 3;
     const bool writer = blockIdx.x == 0;
-    // node-chunk sums (fixed order, one wave per problem) and the exact
-    // fixed-point residual sums
-    for (int p = wave; p < P; p += kWavesPerBlock) {
-        const int fl = s_flags[p];
-        const bool had = just_did_a ? (fl & F_DOA) : (fl & F_DOB);
-        double a = 0.0, b = 0.0;
-        unsigned long long hi = 0ull, lo = 0ull;
-        if (had) {
-            const ProbDesc& D = descs[p];
-            const double* ncp = D.NCP + 2 * (size_t)par * D.nchunks;
-            for (int c = lane; c < D.nchunks; c += kWave) {
-                a += __hip_atomic_load(&ncp[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                b += __hip_atomic_load(&ncp[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        if (just_did_a && lane < kSlots) {
-            hi = __hip_atomic_load(&ws->hi[buf][p][lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            lo = __hip_atomic_load(&ws->lo[buf][p][lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            hi += __shfl_xor(hi, off, kWave);
-            lo += __shfl_xor(lo, off, kWave);
-        }
-        if (lane == 0) {
-            s_fin[p][0] = a;
-            s_fin[p][1] = b;
-            s_sr2[p] = (double)hi + (double)lo / kFixScale;
-            s_badp[p] = just_did_a ? __hip_atomic_load(&ws->bad[buf][p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                   : 0;
-        }
-    }
-    __syncthreads();
     const unsigned long long admit =
         sh ? __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-    for (int p = threadIdx.x; p < P; p += kBlock) {
-        ProbState st = s_st[p];
-        const int prev = st.mode;
-        bool finished = false;
-        int fl;
-        if (prev == MODE_COPY) {  // the copy pass ran in this step
-            st.mode = MODE_DONE;
-            fl = 0;
-            finished = true;
-        } else {
-            fl = decide(descs[p], st, just_did_a, s_sr2[p], s_badp[p] != 0, s_fin[p][0], s_fin[p][1], writer);
-            if (prev != MODE_DONE && st.mode == MODE_DONE) {
-                if (sh && st.cur) {
-                    st.mode = MODE_COPY;
-                    fl = F_COPY;
-                } else {
-                    finished = true;
+    // Plain loads: the barrier's acquire fence orders them after every
+    // block's release of the sums.
+    for (int pw = wave; pw < P; pw += kWavesPerBlock) {
+        const int p = __builtin_amdgcn_readfirstlane(pw);
+        const DecideDesc& D = s_dd[p];
+        const int fl = s_flags[p];
+        const bool had = just_did_a ? (fl & F_DOA) != 0 : (fl & F_DOB) != 0;
+        unsigned long long hi = 0ull, lo = 0ull;
+        int bad = 0;
+        if (just_did_a) {
+            if (lane < kSlots) {
+                hi = gld(&ws->hi[buf][p][0])[lane];
+                lo = gld(&ws->lo[buf][p][0])[lane];
+            }
+            if (lane == kWave - 1) bad = gld(&ws->bad[buf][p])[0];
+        }
+        // node-chunk sums: chunk c = lane + 64 j added in j order
+        double a = 0.0, b = 0.0;
+        if (had) {
+            const int nch = D.nchunks;
+            const D2IC_GLOBAL d2v* ncp = gld(reinterpret_cast<const d2v*>(D.NCP)) + (size_t)par * nch;
+            for (int c0 = 0; c0 < nch; c0 += 4 * kWave) {
+                d2v v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int c = c0 + u * kWave + lane;
+                    v[u] = c < nch ? ncp[c] : d2v{0.0, 0.0};
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a += v[u].x;
+                    b += v[u].y;
                 }
             }
         }
-        if (sh && ((admit >> p) & 1ull)) {
-            // a served request joins: first evaluation at u0 on the next A-step
-            ProbState z;
-            z.mode = MODE_INIT;
-            z.k = z.trial = z.restart = z.final_eval = z.cur = z.dcur = 0;
-            z.alpha = z.beta = z.gnorm = z.Jtrial = 0.f;
-            z.J = z.gg = 0.0;
-            const bool next_is_a = just_did_a ? false : true;
-            z.passes = next_is_a ? 1 : 0;
-            fl = next_is_a ? (F_DOA | F_INIT) : 0;
-            st = z;
-            finished = false;
+        a = wave_sum63(a);
+        b = wave_sum63(b);
+        hi = wave_sum63(hi);
+        lo = wave_sum63(lo);
+        if (lane == kWave - 1) {
+            const double sr2 = (double)hi + (double)lo / kFixScale;
+            ProbState st = s_st[p];
+            const int prev = st.mode;
+            bool finished = false;
+            int fl2;
+            if (prev == MODE_COPY) {  // the copy pass ran in this step
+                st.mode = MODE_DONE;
+                fl2 = 0;
+                finished = true;
+            } else {
+                fl2 = decide(D, st, just_did_a, sr2, bad != 0, a, b, writer);
+                if (prev != MODE_DONE && st.mode == MODE_DONE) {
+                    if (sh && st.cur) {
+                        st.mode = MODE_COPY;
+                        fl2 = F_COPY;
+                    } else {
+                        finished = true;
+                    }
+                }
+            }
+            if (sh && ((admit >> p) & 1ull)) {
+                // a served request joins: first evaluation at u0 on the next A-step
+                ProbState z;
+                z.mode = MODE_INIT;
+                z.k = z.trial = z.restart = z.final_eval = z.cur = z.dcur = 0;
+                z.alpha = z.beta = z.gnorm = z.Jtrial = 0.f;
+                z.J = z.gg = 0.0;
+                const bool next_is_a = just_did_a ? false : true;
+                z.passes = next_is_a ? 1 : 0;
+                fl2 = next_is_a ? (F_DOA | F_INIT) : 0;
+                st = z;
+                finished = false;
+            }
+            if (finished && writer && sh) {
+                if (D.pass_count) *D.pass_count = st.passes;
+                sh->done_iters[p] = st.k;
+                sh->done_passes[p] = st.passes;
+                __threadfence_system();
+                __hip_atomic_store(&sh->done_gen[p],
+                                   __hip_atomic_load(&sh->admitted_gen[p], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_SYSTEM),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            s_st[p] = st;
+            if (writer) S[p] = st;
+            s_flags[p] = fl2;
+            s_alpha[p] = st.alpha;
+            s_beta[p] = st.beta;
         }
-        if (finished && writer && sh) {
-            const ProbDesc& D = descs[p];
-            if (D.pass_count) *D.pass_count = st.passes;
-            sh->done_iters[p] = st.k;
-            sh->done_passes[p] = st.passes;
-            __threadfence_system();
-            __hip_atomic_store(&sh->done_gen[p],
-                               __hip_atomic_load(&sh->admitted_gen[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-        s_st[p] = st;
-        if (writer) S[p] = st;
-        s_flags[p] = fl;
-        s_alpha[p] = st.alpha;
-        s_beta[p] = st.beta;
     }
+    __syncthreads();
     if (writer) {
         const int zb = (step + 2) % 3;
         for (int i = threadIdx.x; i < P * kSlots; i += kBlock) {
@@ -781,15 +818,20 @@ __device__ void block_decide(const ProbDesc* descs, int P, ProbState* __restrict
         for (int p = threadIdx.x; p < P; p += kBlock)
             __hip_atomic_store(&ws->bad[zb][p], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
+    // block-local residual accumulators of the next A-step
+    for (int p = threadIdx.x; p < P; p += kBlock) {
+        s_hi[p] = 0ull;
+        s_lo[p] = 0ull;
+        s_bad[p] = 0;
+    }
     if (threadIdx.x == 0) {
         int to = 0, co = 0, any = 0;
         for (int p = 0; p < P; ++p) {
             const int fl = s_flags[p];
             s_toff[p] = to;
             s_coff[p] = co;
-            if (fl & F_DOA) to += descs[p].ntiles;
-            if (fl & (F_DOA | F_DOB | F_COPY)) co += descs[p].nchunks;
+            if (fl & F_DOA) to += s_dd[p].ntiles;
+            if (fl & (F_DOA | F_DOB | F_COPY)) co += s_dd[p].nchunks;
             if (s_st[p].mode != MODE_DONE) any = 1;  // a problem whose trial was rejected idles one step
         }
         s_hdr[0] = to;
@@ -858,8 +900,8 @@ __device__ void serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
 }
 
 // Served mode, every block after the barrier: copy the descriptors of the
-// slots admitted at this barrier into the block's LDS table.
-__device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, Workspace* ws) {
+// slots admitted at this barrier into the block's LDS tables.
+__device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, DecideDesc* s_dd, Workspace* ws) {
     constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
     const unsigned long long admit = __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (admit) {
@@ -869,6 +911,8 @@ __device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, Workspace
                 reinterpret_cast<unsigned long long*>(s_desc)[i] = __hip_atomic_load(
                     reinterpret_cast<const unsigned long long*>(gdesc) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __syncthreads();
+        if (threadIdx.x < kMaxSlots && ((admit >> threadIdx.x) & 1ull)) fill_decide_desc(s_dd[threadIdx.x], s_desc[threadIdx.x]);
         __syncthreads();
     }
 }
@@ -901,9 +945,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ unsigned long long s_hi[kMaxProblems];
     __shared__ unsigned long long s_lo[kMaxProblems];
     __shared__ int s_bad[kMaxProblems];
-    __shared__ double s_fin[kMaxProblems][2];
-    __shared__ double s_sr2[kMaxProblems];
-    __shared__ int s_badp[kMaxProblems];
+    __shared__ DecideDesc s_dd[kServe ? kMaxSlots : kMaxProblems];
     __shared__ ProbState s_st[kMaxProblems];
     __shared__ int s_hdr[3];
     __shared__ int s_ok;
@@ -936,6 +978,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         s_toff[p] = e.tile_off;
         s_coff[p] = e.chunk_off;
         s_st[p] = S[p];
+        fill_decide_desc(s_dd[p], gdesc[p]);
+        s_hi[p] = 0ull;
+        s_lo[p] = 0ull;
+        s_bad[p] = 0;
     }
     if (threadIdx.x == 0) {
         s_hdr[0] = ws->n_tiles_active;
@@ -952,12 +998,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         const int par = step & 1, buf = step % 3;
         if (!sh && !s_hdr[2]) break;
         const int nT = s_hdr[0], nC = s_hdr[1];
-        for (int p = threadIdx.x; p < P; p += kBlock) {
-            s_hi[p] = 0ull;
-            s_lo[p] = 0ull;
-            s_bad[p] = 0;
-        }
-        __syncthreads();
 
         // ---- node items, one block each (A: d, g.d, spring energy; B: gradient, dots) ----
         for (int c = (int)vb; c < nC; c += nb) {
@@ -1151,9 +1191,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         __syncthreads();
         if (!s_ok) break;
         if (sh && __hip_atomic_load(&ws->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if constexpr (kServe) serve_refresh(gdesc, s_desc, ws);
-        block_decide(descs, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_fin, s_sr2,
-                     s_badp);
+        if constexpr (kServe) serve_refresh(gdesc, s_desc, s_dd, ws);
+        block_decide(s_dd, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_hi, s_lo,
+                     s_bad);
     }
     if (sh && blockIdx.x == 0 && threadIdx.x == 0) {
         __threadfence_system();
