@@ -143,6 +143,10 @@ struct PlanEntry {
 
 constexpr int kMaxBlocks = 1024;
 constexpr int kArriveStride = 32;  // one 128-byte line per block's arrival flag
+// release word: step + 1 (mod 2^30) in the low bits, abort and session-stop flags above
+constexpr unsigned kReleaseAbort = 0x80000000u;
+constexpr unsigned kReleaseStop = 0x40000000u;
+constexpr unsigned kReleaseStepMask = 0x3fffffffu;
 
 // Mailbox of a served session, in pinned host memory (see d2ic_serve_*).
 // The host fills desc[i], then raises submit_gen[i]; the kernel admits the
@@ -844,10 +848,10 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
 // Served mode, master wave (block 0, wave 0), after every block has arrived:
 // admit the requests waiting in the mailbox into free slots (their
 // descriptors are copied to the device slot table) and decide whether the
-// session ends.  Results go to ws->admit_mask / ws->stop, which every block
+// session ends.  Results go to ws->admit_mask / ws->stop (also returned), which every block
 // reads after the barrier.  A slot is free when its problem is done in the
 // (identical) block states.
-__device__ void serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh, const ProbState* s_st,
+__device__ bool serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh, const ProbState* s_st,
                             bool any_active, unsigned long long bell, int lane, unsigned long long& seen_bell,
                             uint64_t& idle_t0) {
     unsigned long long mask = 0ull;
@@ -897,6 +901,7 @@ __device__ void serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
         __hip_atomic_store(&ws->stop, stop ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __threadfence();
+    return stop;
 }
 
 // Served mode, every block after the barrier: copy the descriptors of the
@@ -949,6 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ ProbState s_st[kMaxProblems];
     __shared__ int s_hdr[3];
     __shared__ int s_ok;
+    __shared__ int s_stop;
     __shared__ double s_red[2 * kWavesPerBlock];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
     __attribute__((address_space(3))) float* my_box = (__attribute__((address_space(3))) float*)s_box[threadIdx.x / kWave];
@@ -1122,13 +1128,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         }
 
         // ---- grid barrier: every block publishes its arrival; block 0's first
-        //      wave waits for all of them and releases the step ----
+        //      wave waits for all of them and releases the step.  A release
+        //      word carries the step and the abort / session-stop bits. ----
         __syncthreads();
         if (threadIdx.x == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             s_ok = 1;
+            s_stop = 0;
         }
         if (blockIdx.x == 0) {
             if (wave == 0) {
@@ -1139,12 +1147,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 unsigned long long bell = 0ull;
                 if (sh) bell = __hip_atomic_load(&sh->doorbell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 while (true) {
+                    // every load of one poll is in flight at once
+                    const int err = __hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     bool mine = true;
-                    for (int b = lane; b < nb; b += kWave)
-                        mine = mine && __hip_atomic_load(&ws->arrive[b * kArriveStride], __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT) > (unsigned)step;
+                    for (int b0 = 0; b0 < nb; b0 += 4 * kWave) {
+                        unsigned v[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int b = b0 + j * kWave + lane;
+                            v[j] = b < nb ? __hip_atomic_load(&ws->arrive[b * kArriveStride], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : (unsigned)(step + 1);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) mine = mine & (v[j] == (unsigned)(step + 1));
+                    }
                     if (__all(mine)) break;
-                    if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (err) {
                         ok = 0;
                         break;
                     }
@@ -1160,37 +1179,49 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     __builtin_amdgcn_s_sleep(1);
                 }
                 __threadfence();
+                bool stop = false;
                 if constexpr (kServe)
-                    serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
+                    stop = serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
+                const unsigned word =
+                    ((unsigned)(step + 1) & kReleaseStepMask) | (ok ? 0u : kReleaseAbort) | (stop ? kReleaseStop : 0u);
                 for (int b = lane; b < nb; b += kWave)
-                    __hip_atomic_store(&ws->release[b * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
+                    __hip_atomic_store(&ws->release[b * kArriveStride], word, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
-                if (lane == 0 && !ok) s_ok = 0;
+                if (lane == 0) {
+                    if (!ok) s_ok = 0;
+                    if (stop) s_stop = 1;
+                }
             }
         } else if (threadIdx.x == 0) {
             const uint64_t tstart = wall_clock64();
-            unsigned polls = 0;
-            while (__hip_atomic_load(&ws->release[blockIdx.x * kArriveStride], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) <= (unsigned)step) {
-                if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    s_ok = 0;
+            unsigned polls = 0, word = 0;
+            while (true) {
+                word = __hip_atomic_load(&ws->release[blockIdx.x * kArriveStride], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+                const int err = __hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // the word holds this step or the one before it (the release of
+                // the next step needs this block's arrival), so equality is exact
+                if ((word & kReleaseStepMask) == ((unsigned)(step + 1) & kReleaseStepMask)) break;
+                if (err) {
+                    word = kReleaseAbort;
                     break;
                 }
                 ++polls;
                 if (polls > kBarrierMinPolls && wall_clock64() - tstart > kBarrierTimeoutTicks) {
                     atomicExch(&ws->err, 1);
                     printf("d2ic: grid barrier timeout (block %u, step %d)\n", blockIdx.x, step);
-                    s_ok = 0;
+                    word = kReleaseAbort;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) s_ok = 0;
+            if (word & kReleaseAbort) s_ok = 0;
+            if (word & kReleaseStop) s_stop = 1;
         }
         __syncthreads();
         if (!s_ok) break;
-        if (sh && __hip_atomic_load(&ws->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (s_stop) break;
         if constexpr (kServe) serve_refresh(gdesc, s_desc, s_dd, ws);
         block_decide(s_dd, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_hi, s_lo,
                      s_bad);
