@@ -519,11 +519,13 @@ __device__ __forceinline__ void block_store2(double a, double b, double* s_red, 
 }
 
 // A-step node item (128 nodes, 8 lanes per node): materialise d, g.d and the
-// spring energy at the candidate (both halves of every ordered pair).
+// spring energy at the candidate (both halves of every ordered pair).  Loads
+// go out in two rounds: the node's own values with the neighbour list, then
+// the neighbours' values (further neighbours, past 8, in later rounds).
 __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
                             double* s_red, int par) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
-    const bool restart = (flags & F_RESTART) != 0;
+    const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int sub = lane % kLanesPerNode;
     const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
@@ -532,21 +534,44 @@ __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags,
         const float* U = D->U[cur];
         const float* G = D->G[cur];
         const float* Dp = D->D[dcur];
-        const bool init = (flags & F_INIT) != 0;
+        const int md = D->nb_maxdeg;
+        int j = -1;
+        float w = 0.f;
+        if (sub < md) {
+            j = gld(D->nb_idx)[(size_t)n * md + sub];
+            w = gld(D->nb_wp)[(size_t)n * md + sub];
+        }
+        const float2 un = ld2(U, n);
         const float2 g = init ? make_float2(0.f, 0.f) : ld2(G, n);
-        const float2 d = init ? make_float2(0.f, 0.f) : direction(g, ld2(Dp, n), beta, restart);
+        const float2 dp = init ? make_float2(0.f, 0.f) : ld2(Dp, n);
+        float2 uj = make_float2(0.f, 0.f), gj = uj, dj = uj;
+        if (j >= 0) {
+            uj = ld2(U, j);
+            if (!init) {
+                gj = ld2(G, j);
+                dj = ld2(Dp, j);
+            }
+        }
+        const float2 d = init ? make_float2(0.f, 0.f) : direction(g, dp, beta, restart);
         if (sub == 0) {
             st2_wt(D->D[dcur ^ 1], n, d);
             gd = (double)g.x * (double)d.x + (double)g.y * (double)d.y;
         }
-        const float2 ucn = step_to(ld2(U, n), d, alpha);
-        const int md = D->nb_maxdeg;
+        const float2 ucn = step_to(un, d, alpha);
         for (int k = sub; k < md; k += kLanesPerNode) {
-            const int j = gld(D->nb_idx)[(size_t)n * md + k];
-            if (j < 0) continue;
-            const float w = gld(D->nb_wp)[(size_t)n * md + k];
-            const float2 ucj =
-                init ? ld2(U, j) : step_to(ld2(U, j), direction(ld2(G, j), ld2(Dp, j), beta, restart), alpha);
+            if (k != sub) {
+                j = gld(D->nb_idx)[(size_t)n * md + k];
+                if (j < 0) continue;
+                w = gld(D->nb_wp)[(size_t)n * md + k];
+                uj = ld2(U, j);
+                if (!init) {
+                    gj = ld2(G, j);
+                    dj = ld2(Dp, j);
+                }
+            } else if (j < 0) {
+                continue;
+            }
+            const float2 ucj = init ? uj : step_to(uj, direction(gj, dj, beta, restart), alpha);
             const float dx = ucn.x - ucj.x, dy = ucn.y - ucj.y;
             E += (double)(w * (dx * dx + dy * dy));
         }
@@ -565,7 +590,10 @@ __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
 }
 
 // B-step node item: new point, gradient gather, |g|^2, g.(g - g_prev), and
-// the scatter of (u, g, d) into the tile records for the next A-step.
+// the scatter of (u, g, d) into the tile records for the next A-step.  Loads
+// go out in three rounds: topology and the node's own values; the first tile
+// entry and neighbour values; the entry's tile partials.  Entries and
+// neighbours past the first 8 follow in order.
 __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, double* s_red,
                             int par) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
@@ -574,31 +602,54 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
     const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
     const bool valid = n < D->nnodes;
     double gg = 0.0, gnum = 0.0;
-    float2 un = make_float2(0.f, 0.f), dn = make_float2(0.f, 0.f), go = make_float2(0.f, 0.f);
+    float2 un = make_float2(0.f, 0.f), go = make_float2(0.f, 0.f);
     double sx = 0.0, sy = 0.0;
     float rx = 0.f, ry = 0.f;
-    int e0 = 0, e1 = 0;
     const float* U = D->U[cur];
     const float* Dn = D->D[dcur ^ 1];
     if (valid) {
-        dn = ld2(Dn, n);
-        un = step_to(ld2(U, n), dn, alpha);
+        const int md = D->nb_maxdeg;
+        const int e0 = gld(D->node_ptr)[n];
+        const int e1 = gld(D->node_ptr)[n + 1];
+        int j = -1;
+        float w = 0.f;
+        if (sub < md) {
+            j = gld(D->nb_idx)[(size_t)n * md + sub];
+            w = gld(D->nb_wp)[(size_t)n * md + sub];
+        }
+        const float2 dn = ld2(Dn, n);
+        const float2 u0 = ld2(U, n);
         go = ld2(D->G[cur], n);
-        e0 = gld(D->node_ptr)[n];
-        e1 = gld(D->node_ptr)[n + 1];
-        for (int e = e0 + sub; e < e1; e += kLanesPerNode) {
+        const int ent = e0 + sub < e1 ? gld(D->node_ent)[e0 + sub] : -1;
+        float2 uj = make_float2(0.f, 0.f), dj = uj;
+        if (j >= 0) {
+            uj = ld2(U, j);
+            dj = ld2(Dn, j);
+        }
+        if (ent >= 0) {
+            const float2 q = ld2(D->TP, ent);
+            sx += (double)q.x;
+            sy += (double)q.y;
+        }
+        for (int e = e0 + sub + kLanesPerNode; e < e1; e += kLanesPerNode) {
             const float2 q = ld2(D->TP, gld(D->node_ent)[e]);
             sx += (double)q.x;
             sy += (double)q.y;
         }
-        const int md = D->nb_maxdeg;
+        un = step_to(u0, dn, alpha);
         for (int k = sub; k < md; k += kLanesPerNode) {
-            const int j = gld(D->nb_idx)[(size_t)n * md + k];
-            if (j < 0) continue;
-            const float w = gld(D->nb_wp)[(size_t)n * md + k];
-            const float2 uj = step_to(ld2(U, j), ld2(Dn, j), alpha);
-            rx += w * (un.x - uj.x);
-            ry += w * (un.y - uj.y);
+            if (k != sub) {
+                j = gld(D->nb_idx)[(size_t)n * md + k];
+                if (j < 0) continue;
+                w = gld(D->nb_wp)[(size_t)n * md + k];
+                uj = ld2(U, j);
+                dj = ld2(Dn, j);
+            } else if (j < 0) {
+                continue;
+            }
+            const float2 ujn = step_to(uj, dj, alpha);
+            rx += w * (un.x - ujn.x);
+            ry += w * (un.y - ujn.y);
         }
     }
     sx = group_sum(sx);
