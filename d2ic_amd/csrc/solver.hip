@@ -496,7 +496,10 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-// Block-level sum of two per-lane values in fixed order; thread 0 stores them.
+// Block-level sum of two per-lane values in fixed order: wave sums through
+// DPP, then the 16 wave partials summed across the first row of wave 0
+// (quad_perm, row_half_mirror, row_mirror); lane 0 stores them.
+static_assert(kWavesPerBlock == 16, "block_store2 folds one 16-lane row");
 __device__ __forceinline__ void block_store2(double a, double b, double* s_red, double* out) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     a = wave_sum63(a);
@@ -506,14 +509,21 @@ __device__ __forceinline__ void block_store2(double a, double b, double* s_red, 
         s_red[2 * wave + 1] = b;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = 0.0, y = 0.0;
-        for (int w = 0; w < kWavesPerBlock; ++w) {
-            x += s_red[2 * w];
-            y += s_red[2 * w + 1];
+    if (wave == 0) {
+        double x = lane < kWavesPerBlock ? s_red[2 * lane] : 0.0;
+        double y = lane < kWavesPerBlock ? s_red[2 * lane + 1] : 0.0;
+        x += dpp_d<0xB1>(x);
+        y += dpp_d<0xB1>(y);
+        x += dpp_d<0x4E>(x);
+        y += dpp_d<0x4E>(y);
+        x += dpp_d<0x141>(x);
+        y += dpp_d<0x141>(y);
+        x += dpp_d<0x140>(x);
+        y += dpp_d<0x140>(y);
+        if (lane == 0) {
+            st_wt(out, x);
+            st_wt(out + 1, y);
         }
-        st_wt(out, x);
-        st_wt(out + 1, y);
     }
     __syncthreads();
 }
