@@ -983,6 +983,13 @@ __device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, DecideDes
     }
 }
 
+// First node item of block vb (XCD-aware block order): items are dealt in
+// contiguous runs, ceil(vb nC / nb), so that blocks with node work are spread
+// over every XCD.
+__device__ __forceinline__ int item_begin(int vb, int nC, int nb) {
+    return (int)(((long long)vb * nC + nb - 1) / nb);
+}
+
 // wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
 // a wait is abandoned only after both 2^20 polls and 4 s.
 constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
@@ -1066,8 +1073,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         if (!sh && !s_hdr[2]) break;
         const int nT = s_hdr[0], nC = s_hdr[1];
 
-        // ---- node items, one block each (A: d, g.d, spring energy; B: gradient, dots) ----
-        for (int c = (int)vb; c < nC; c += nb) {
+        // ---- node items (A: d, g.d, spring energy; B: gradient, dots): block vb
+        //      takes items [ceil(vb nC / nb), ceil((vb + 1) nC / nb)) ----
+        const int ci0 = item_begin(vb, nC, nb), ci1 = item_begin(vb + 1, nC, nb);
+        for (int c = ci0; c < ci1; ++c) {
             const int p = find_prob(s_coff, P, c);
             const ProbDesc* D = descs + p;
             const int fl = s_flags[p];
@@ -1193,7 +1202,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         //      word carries the step and the abort / session-stop bits. ----
         __syncthreads();
         if (threadIdx.x == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            // Only a block that wrote data for other blocks in this step
+            // releases it: a release fence writes back its XCD's L2, and idle
+            // blocks fencing at once would queue those write-backs in front of
+            // the loads of the blocks still working on that XCD.  (An idle
+            // block's loads of this step have all returned: their values were
+            // used before the block arrived.)
+            const bool wrote = blockIdx.x == 0 || (isA && nT > 0) || ci0 < ci1;
+            if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             s_ok = 1;
