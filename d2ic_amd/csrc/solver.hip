@@ -258,16 +258,17 @@ struct TilePix {  // per-lane pixel data of one chunk (f is loaded at use)
     float4 N;
 };
 
-// A tile's wave-wide record, one value per lane: lanes 0..23 hold
-// (u, g, d_prev) of local node a = lane / 6 (component lane % 6), lanes
-// 24..28 the static record.  Built in two dependent loads: tile_ids gives each
-// lane its node id (or static field), tile_vals the value.
+// A tile's wave-wide record, one value per lane: lane l < 24 holds component
+// l % 2 of local node (l % 8) / 2 of u (l < 8), g (8 <= l < 16) or d_prev
+// (16 <= l < 24); lanes 24..28 the static record.  Built in two dependent
+// loads: tile_ids gives each lane its node id (or static field), tile_vals
+// the value.
 typedef int i4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int tile_ids(const ProbDesc* __restrict__ D, int t, int lane) {
     if (lane < 24) {
         const i4v nd = gld(reinterpret_cast<const i4v*>(D->tile_nodes))[t];
-        const int a = lane / 6;
+        const int a = (lane & 7) >> 1;
         return a == 0 ? nd.x : (a == 1 ? nd.y : (a == 2 ? nd.z : nd.w));
     }
     if (lane < 29) return __float_as_int(gld(D->TV)[(size_t)t * kTV + (lane - 24)]);
@@ -276,10 +277,10 @@ __device__ __forceinline__ int tile_ids(const ProbDesc* __restrict__ D, int t, i
 
 __device__ __forceinline__ float tile_vals(const ProbDesc* __restrict__ D, int id, int lane, int flags) {
     if (lane < 24) {
-        const int c = lane % 6;
-        const float* base = c < 2 ? D->U[(flags & F_CUR) ? 1 : 0]
-                                  : (c < 4 ? D->G[(flags & F_CUR) ? 1 : 0] : D->D[(flags & F_DCUR) ? 1 : 0]);
-        return gld(base)[2 * (size_t)id + (c & 1)];
+        const int which = lane >> 3;
+        const float* base = which == 0 ? D->U[(flags & F_CUR) ? 1 : 0]
+                                       : (which == 1 ? D->G[(flags & F_CUR) ? 1 : 0] : D->D[(flags & F_DCUR) ? 1 : 0]);
+        return gld(base)[2 * (size_t)id + (lane & 1)];
     }
     return __int_as_float(id);
 }
@@ -300,21 +301,20 @@ __device__ __forceinline__ float bcast(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-__device__ __forceinline__ float uni(float v) {  // wave-uniform value -> SGPR
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
-// Candidate node displacements of a tile (wave-uniform, from the tile record).
+// Candidate node displacements of a tile (wave-uniform, from the tile record):
+// lanes 0..7 form u + alpha d component-wise (the operations of direction()
+// and step_to()), with g fetched from 8 lanes up (row_ror:8) and d_prev from
+// the next 16-lane row (v_permlane16_swap moves row 1 into row 0), then 8
+// readlanes make them scalar.
 __device__ __forceinline__ void tile_uc(float tv, int flags, float alpha, float beta, float2 (&uc)[4]) {
     const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
+    const float g = dpp_f<0x128>(tv);
+    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(tv), __float_as_uint(tv), false, false);
+    const float dp = __uint_as_float(sw[1]);
+    const float d = restart ? -g : __fmaf_rn(beta, dp, -g);
+    const float c = init ? tv : __fmaf_rn(alpha, d, tv);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const float2 u = make_float2(bcast(tv, 6 * a), bcast(tv, 6 * a + 1));
-        const float2 gr = make_float2(bcast(tv, 6 * a + 2), bcast(tv, 6 * a + 3));
-        const float2 dp = make_float2(bcast(tv, 6 * a + 4), bcast(tv, 6 * a + 5));
-        const float2 c = init ? u : step_to(u, direction(gr, dp, beta, restart), alpha);
-        uc[a] = make_float2(uni(c.x), uni(c.y));
-    }
+    for (int a = 0; a < 4; ++a) uc[a] = make_float2(bcast(c, 2 * a), bcast(c, 2 * a + 1));
 }
 
 // Pixel count and the box of image rows/columns a tile's samples can touch:
@@ -1132,8 +1132,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     // loads; values of tile it+2 and node ids of tile it+3 requested
                     box_store(b_cur, lane, R, my_box);
                     TileBox b_nxt;
+                    float2 ucn[4];
                     if (it + 1 < t1) {
-                        float2 ucn[4];
                         tile_uc(tv_n1, s_flags[pb], s_alpha[pb], s_beta[pb], ucn);
                         b_nxt = tile_box(tv_n1, ucn);
                         box_load(descs + pb, b_nxt, lane, R);
@@ -1177,7 +1177,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                         hi = lo = 0ull;
                         bad = 0;
                     }
-                    if (it + 1 < t1) tile_uc(tv_n1, s_flags[pb], s_alpha[pb], s_beta[pb], uc);
+                    if (it + 1 < t1) {
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) uc[a] = ucn[a];
+                    }
                     pa = pb;
                     pb = pc;
                     pc = pd;
