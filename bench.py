@@ -318,20 +318,29 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
     act = plan.node_active
     win = torch.tensor((np.abs(X[:, 0] - W / 2) <= 100) & (np.abs(X[:, 1] - H / 2) <= 100), device="cuda") & act
     u_target = torch.tensor(np.stack([ux, uy], 1), dtype=torch.float32, device="cuda")
-    f = ref.clone().requires_grad_(True)
-    gg = g.clone().requires_grad_(True)
-    alpha = torch.tensor(reg, device="cuda", requires_grad=True)
-    torch.cuda.synchronize()
+
+    def forward_backward():
+        f = ref.clone().requires_grad_(True)
+        gg = g.clone().requires_grad_(True)
+        alpha = torch.tensor(reg, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u = solve_differentiable(plan, f, gg, alpha, max_iter=max_iters, tol=tol, cg_rtol=1e-6)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        E = green_lagrange_torch(u, X, nb_idx, nb_deg)
+        loss = ((u - u_target)[act] ** 2).sum() + E[win, 0, 0].mean()
+        t0 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        return f, gg, alpha, u, loss, t_fwd, time.perf_counter() - t0
+
+    # the first call pays one-time costs (lazy loading of torch's autograd
+    # kernels); the second is timed
     t0 = time.perf_counter()
-    u = solve_differentiable(plan, f, gg, alpha, max_iter=max_iters, tol=tol, cg_rtol=1e-6)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    E = green_lagrange_torch(u, X, nb_idx, nb_deg)
-    loss = ((u - u_target)[act] ** 2).sum() + E[win, 0, 0].mean()
-    t0 = time.perf_counter()
-    loss.backward()
-    torch.cuda.synchronize()
-    t_bwd = time.perf_counter() - t0
+    forward_backward()
+    t_first = time.perf_counter() - t0
+    f, gg, alpha, u, loss, t_fwd, t_bwd = forward_backward()
     lin = Linearization(plan, g, u.detach())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -344,17 +353,17 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
         noise = torch.randn((nv, H, W), generator=gen, device="cuda")
         du = lin.tangent(noise, reg, "g", rtol=1e-4)
         du_sq += (du.double() ** 2).sum(0)
-        # first-order E_xx change along each probe (central difference around u*)
-        for k in range(nv):
-            e1 = green_lagrange_torch(u.detach() + 1e-3 * du[k], X, nb_idx, nb_deg)[:, 0, 0]
-            e0 = green_lagrange_torch(u.detach() - 1e-3 * du[k], X, nb_idx, nb_deg)[:, 0, 0]
-            exx_sq += (((e1 - e0) / 2e-3).double() ** 2)
+        # first-order E_xx change along each probe (central difference around u*),
+        # all probes of the batch at once
+        e1 = green_lagrange_torch(u.detach()[None] + 1e-3 * du, X, nb_idx, nb_deg)[..., 0, 0]
+        e0 = green_lagrange_torch(u.detach()[None] - 1e-3 * du, X, nb_idx, nb_deg)[..., 0, 0]
+        exx_sq += (((e1 - e0) / 2e-3).double() ** 2).sum(0)
     torch.cuda.synchronize()
     t_probe = time.perf_counter() - t0
     u_std = torch.sqrt(du_sq / n_probes)[act]
     exx_std = torch.sqrt(exx_sq / n_probes)[win]
     res = dict(image=[H, W], n_pixels=plan.n_pixels, n_nodes=plan.n_nodes, loss=float(loss.detach()),
-               forward_solve_s=t_fwd, backward_s=t_bwd, probes=n_probes, probes_s=t_probe,
+               forward_solve_s=t_fwd, backward_s=t_bwd, first_call_s=t_first, probes=n_probes, probes_s=t_probe,
                grad_norms=dict(ref=float(f.grad.norm()), deformed=float(gg.grad.norm()), reg=float(alpha.grad)),
                u_std_median_px=float(u_std.median()), exx_std_window_median=float(exx_std.median()),
                check="finite differences of FP64 oracle solves in tests/test_gpu_adjoint.py (rel. err ~2e-4)")

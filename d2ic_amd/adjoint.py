@@ -68,29 +68,56 @@ class Linearization:
                   _lib.stream())
         return out
 
-    def solve(self, B, alpha, rtol: float = 1e-6, max_iter: int = 2000, check_every: int = 10):
-        """Conjugate gradients on H X = B for every right-hand side of B (nv, n_nodes, 2);
-        FP64 dot products, per-RHS step sizes."""
+    def precond(self, alpha):
+        """Block-Jacobi preconditioner: the inverse 2x2 diagonal block of H per
+        node as two (n_nodes, 2) factors (A, C) with M^-1 r = A * r + C * flip(r)."""
+        t = _lib.torch()
+        if getattr(self, "_pc_alpha", None) != float(alpha):
+            inv = t.empty((self.plan.n_nodes, 3), dtype=t.float32, device=_lib.device())
+            _lib.call("d2ic_adj_diag_inv", self.plan.handle, _lib.ptr(self.K), float(alpha), _lib.ptr(inv),
+                      _lib.stream())
+            inv = inv * self.mask
+            self._pc = (t.stack([inv[:, 0], inv[:, 2]], 1).contiguous(), t.stack([inv[:, 1], inv[:, 1]], 1).contiguous())
+            self._pc_alpha = float(alpha)
+        return self._pc
+
+    def solve(self, B, alpha, rtol: float = 1e-6, max_iter: int = 2000, check_every: int = 10,
+              precondition: bool = True):
+        """Conjugate gradients on H X = B for every right-hand side of B (nv, n_nodes, 2),
+        block-Jacobi preconditioned unless ``precondition=False``; FP64 dot
+        products, per-RHS step sizes; stops when every |R| <= rtol |B|
+        (tested every ``check_every`` iterations)."""
         t = _lib.torch()
         m = self.mask
         B = B * m
         X = t.zeros_like(B)
         R = B.clone()
-        P = R.clone()
-        rr = (R.double() ** 2).sum(dim=(1, 2))
-        stop = (rtol ** 2) * rr.clamp_min(1e-300)
+        if precondition:
+            A, C = self.precond(alpha)
+
+            def apply(V):
+                return t.addcmul(A * V, C, V.flip(-1))
+        else:
+            def apply(V):
+                return V.clone()
+        Z = apply(R)
+        P = Z.clone()
+        rz = (R.double() * Z.double()).sum(dim=(1, 2))
+        stop = (rtol ** 2) * (R.double() ** 2).sum(dim=(1, 2)).clamp_min(1e-300)
         for it in range(max_iter):
             HP = self.hvp(P, alpha) * m
             php = (P.double() * HP.double()).sum(dim=(1, 2))
-            a = t.where(php > 0, rr / php.clamp_min(1e-300), t.zeros_like(rr))
-            X = X + a.float()[:, None, None] * P
-            R = R - a.float()[:, None, None] * HP
-            rr_new = (R.double() ** 2).sum(dim=(1, 2))
-            if (it + 1) % check_every == 0 and bool((rr_new <= stop).all()):
+            a = t.where(php > 0, rz / php.clamp_min(1e-300), t.zeros_like(rz)).float()[:, None, None]
+            X.addcmul_(a, P)
+            R.addcmul_(a, HP, value=-1.0)
+            if (it + 1) % check_every == 0 and bool(((R.double() ** 2).sum(dim=(1, 2)) <= stop).all()):
                 break
-            beta = t.where(rr > 0, rr_new / rr.clamp_min(1e-300), t.zeros_like(rr))
-            P = R + beta.float()[:, None, None] * P
-            rr = rr_new
+            Z = apply(R)
+            rz_new = (R.double() * Z.double()).sum(dim=(1, 2))
+            beta = t.where(rz > 0, rz_new / rz.clamp_min(1e-300), t.zeros_like(rz)).float()[:, None, None]
+            P = t.addcmul(Z, beta, P)
+            rz = rz_new
+        self.last_iters = it + 1
         return X
 
     def tangent_rhs(self, dimg, which: str = "g"):
@@ -186,7 +213,7 @@ def solve_differentiable(plan: CGPlan, ref_image, def_image, reg_strength, u0=No
 
 def green_lagrange_torch(u, nodes, nb_index, nb_degree, gauge_length: float = 0.0, eps: float = 1e-8):
     """Nodal LSQ Green-Lagrange tensor (``strain.py:11-114``) as differentiable
-    torch code: returns E (n_nodes, 2, 2)."""
+    torch code: returns E (..., n_nodes, 2, 2) for u (..., n_nodes, 2)."""
     t = _lib.torch()
     X = _lib.to_dev(nodes, t.float32).reshape(-1, 2).to(u.dtype)
     idx = _lib.to_dev(nb_index, t.int64)
@@ -194,17 +221,17 @@ def green_lagrange_torch(u, nodes, nb_index, nb_degree, gauge_length: float = 0.
     md = idx.shape[1]
     valid = (t.arange(md, device=idx.device)[None, :] < deg[:, None])
     dX = X[idx] - X[:, None, :]
-    dU = u[idx] - u[:, None, :]
+    dU = u[..., idx, :] - u[..., :, None, :]
     if gauge_length > 0:
         w = t.exp(-((dX ** 2).sum(-1)) / (gauge_length ** 2))
     else:
         w = t.ones(dX.shape[:2], dtype=u.dtype, device=u.device)
     w = w * valid.to(u.dtype)
     A = t.einsum("nk,nki,nkj->nij", w, dX, dX) + eps * t.eye(2, dtype=u.dtype, device=u.device)
-    Bm = t.einsum("nk,nki,nkj->nij", w, dU, dX)  # rows: component of u, cols: direction
-    grad_u = t.linalg.solve(A, Bm.transpose(1, 2)).transpose(1, 2)
+    Bm = t.einsum("nk,...nki,nkj->...nij", w, dU, dX)  # rows: component of u, cols: direction
+    grad_u = t.linalg.solve(A, Bm.transpose(-1, -2)).transpose(-1, -2)
     F = t.eye(2, dtype=u.dtype, device=u.device) + grad_u
-    return 0.5 * (F.transpose(1, 2) @ F - t.eye(2, dtype=u.dtype, device=u.device))
+    return 0.5 * (F.transpose(-1, -2) @ F - t.eye(2, dtype=u.dtype, device=u.device))
 
 
 def strain_noise_std(lin: Linearization, alpha, n_probes: int = 256, sigma: float = 1.0, seed: int = 0,

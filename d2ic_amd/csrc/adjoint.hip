@@ -160,6 +160,46 @@ __global__ void k_adj_gather(PlanView pl, const float* __restrict__ TPV, int nv,
     out[(size_t)k * pl.nn * 2 + 2 * n + 1] = (float)(sy * scale) + alpha * ry;
 }
 
+// Inverse 2 x 2 diagonal blocks of H (block-Jacobi preconditioner): for node n,
+// B = scale * sum over its (tile, slot a) entries of the tile's (a, a) block
+// + alpha * sum_j w_nj * I, in table order; stored as (ixx, ixy, iyy).  A
+// block that is not positive definite (the r hess g term can make it so)
+// falls back to 1 / (|Bxx| + |Byy|) * I; a zero block gives 0.
+__global__ void k_adj_diag_inv(PlanView pl, const float* __restrict__ K, double scale, float alpha,
+                               float* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= pl.nn) return;
+    double bxx = 0.0, bxy = 0.0, byy = 0.0;
+    for (int e = pl.node_ptr[n]; e < pl.node_ptr[n + 1]; ++e) {
+        const int ent = pl.node_ent[e];
+        const int t = ent >> 2, a = ent & 3;
+        const float* Kt = K + (int64_t)t * kK + 3 * pair_index(a, a);
+        bxx += (double)Kt[0];
+        bxy += (double)Kt[1];
+        byy += (double)Kt[2];
+    }
+    double sw = 0.0;
+    for (int j = 0; j < pl.nb_maxdeg; ++j) {
+        const int m = pl.nb_idx[(size_t)n * pl.nb_maxdeg + j];
+        if (m >= 0) sw += (double)pl.nb_wp[(size_t)n * pl.nb_maxdeg + j];
+    }
+    bxx = bxx * scale + (double)alpha * sw;
+    byy = byy * scale + (double)alpha * sw;
+    bxy = bxy * scale;
+    const double det = bxx * byy - bxy * bxy, tr = fabs(bxx) + fabs(byy);
+    float ixx = 0.f, ixy = 0.f, iyy = 0.f;
+    if (bxx > 0.0 && det > 1e-12 * tr * tr) {
+        ixx = (float)(byy / det);
+        ixy = (float)(-bxy / det);
+        iyy = (float)(bxx / det);
+    } else if (tr > 0.0) {
+        ixx = iyy = (float)(1.0 / tr);
+    }
+    out[3 * (size_t)n] = ixx;
+    out[3 * (size_t)n + 1] = ixy;
+    out[3 * (size_t)n + 2] = iyy;
+}
+
 // Tangent partials per tile and perturbation image: sum_p N_a c_p with
 //   which 0 (g): c_p = dg(x_p) grad g(x_p) + r_p grad dg(x_p)
 //   which 1 (f): c_p = -df(pixel p) grad g(x_p)
@@ -353,6 +393,17 @@ int d2ic_adj_image_grads(const d2ic_cg_plan* plan, int32_t H, int32_t W, const f
     }
     if (grad_g) {
         k_fix_to_float<<<ceil_div(npx, 256), 256, 0, st>>>(work, npx, grad_g);
+        D2IC_LAUNCH_CHECK();
+    }
+    return D2IC_OK;
+}
+
+int d2ic_adj_diag_inv(const d2ic_cg_plan* plan, const float* K, float alpha_reg, float* out, void* stream) {
+    D2IC_CHECK_ARG(plan && K && out, "arguments must not be NULL");
+    hipStream_t st = (hipStream_t)stream;
+    if (plan->nn > 0) {
+        k_adj_diag_inv<<<ceil_div(plan->nn, 256), 256, 0, st>>>(plan_view(plan), K,
+                                                               1.0 / (double)std::max<int64_t>(plan->np, 1), alpha_reg, out);
         D2IC_LAUNCH_CHECK();
     }
     return D2IC_OK;

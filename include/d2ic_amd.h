@@ -181,6 +181,11 @@ int d2ic_adj_tangent(const d2ic_cg_plan* plan, int32_t H, int32_t W, const float
 int d2ic_adj_image_grads(const d2ic_cg_plan* plan, int32_t H, int32_t W, const float* u, const float* lin,
                          const float* lambda, float* grad_f, float* grad_g, unsigned long long* work, void* stream);
 
+/* Inverse 2x2 diagonal blocks of H per node, out (n_nodes,3) = (ixx, ixy, iyy):
+ * a block-Jacobi preconditioner for the linear solves with H.  No counterpart
+ * in the reference (jax.grad differentiates through the unrolled solve). */
+int d2ic_adj_diag_inv(const d2ic_cg_plan* plan, const float* K, float alpha_reg, float* out, void* stream);
+
 /* out[k] = L_W v[k], the spring-energy gradient operator without alpha
  * (dG/dalpha = L_W u); work: nv * tile_floats. */
 int d2ic_adj_spring(const d2ic_cg_plan* plan, const float* v, int32_t nv, float* out, float* work, void* stream);
