@@ -1005,9 +1005,13 @@ constexpr unsigned kBarrierMinPolls = 1u << 20;
 template <bool kServe>
 using DescTable = typename std::conditional<kServe, ProbDesc*, const ProbDesc* __restrict__>::type;
 
-template <bool kServe>
+// kSingle: one problem (P == 1), so every problem lookup folds to 0 and the
+// tile pipeline carries no per-problem state.
+template <bool kServe, bool kSingle = false>
 __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdesc, int P, ProbState* __restrict__ S,
                                                           Workspace* __restrict__ ws, int max_steps, ServeHost* sh) {
+    static_assert(!(kServe && kSingle), "a served session has many slots");
+    if constexpr (kSingle) P = 1;
     // served sessions read descriptors from an LDS copy refreshed on admission
     __shared__ ProbDesc s_desc[kServe ? kMaxSlots : 1];
     __shared__ int s_flags[kMaxProblems];
@@ -1077,7 +1081,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         //      takes items [ceil(vb nC / nb), ceil((vb + 1) nC / nb)) ----
         const int ci0 = item_begin(vb, nC, nb), ci1 = item_begin(vb + 1, nC, nb);
         for (int c = ci0; c < ci1; ++c) {
-            const int p = find_prob(s_coff, P, c);
+            const int p = kSingle ? 0 : find_prob(s_coff, P, c);
             const ProbDesc* D = descs + p;
             const int fl = s_flags[p];
             if (fl & F_COPY)
@@ -1095,8 +1099,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             const int t1 = (int)(((long long)nT * (gwave + 1)) / nwaves);
             if (t0 < t1) {
                 // problem of global tile index it (monotone walk)
-                int pa = find_prob(s_toff, P, t0);
+                int pa = kSingle ? 0 : find_prob(s_toff, P, t0);
                 auto next_prob = [&](int p, int it) {
+                    if constexpr (kSingle) return 0;
                     while (p + 1 < P && it >= s_toff[p + 1]) ++p;
                     return p;
                 };
@@ -1618,8 +1623,9 @@ thread_local Staging g_stage;
 // Co-resident capacity of a persistent-kernel instance (cached per device).
 int occupancy_blocks(const void* kernel, int* out) {
     static std::mutex mu;
-    static int cached_dev[2] = {-1, -1}, cached[2] = {0, 0};
-    const int slot = kernel == (const void*)k_cg_persistent<true> ? 1 : 0;
+    static int cached_dev[3] = {-1, -1, -1}, cached[3] = {0, 0, 0};
+    const int slot = kernel == (const void*)k_cg_persistent<true> ? 1
+                     : (kernel == (const void*)k_cg_persistent<false, true> ? 2 : 0);
     int dev = 0;
     D2IC_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(mu);
@@ -1995,8 +2001,10 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     k_init<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_ws);
     D2IC_LAUNCH_CHECK();
 
+    // a single problem runs the instance without problem lookups
+    const void* kern = P == 1 ? (const void*)k_cg_persistent<false, true> : (const void*)k_cg_persistent<false>;
     int coop_max = 0;
-    int rc = occupancy_blocks((const void*)k_cg_persistent<false>, &coop_max);
+    int rc = occupancy_blocks(kern, &coop_max);
     if (rc != D2IC_OK) return rc;
     // enough blocks for ~4 chunks of 64 pixels per wave, at least one XCD-round
     // (8), at most what can be co-resident; a multiple of 8 keeps the XCD-aware
@@ -2020,11 +2028,13 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         return e && e[0] == '1';
     }();
     if (plain_launch) {
-        k_cg_persistent<false><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        if (P == 1)
+            k_cg_persistent<false, true><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        else
+            k_cg_persistent<false><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
         D2IC_LAUNCH_CHECK();
     } else {
-        D2IC_HIP(hipLaunchCooperativeKernel((const void*)k_cg_persistent<false>, dim3(nblocks), dim3(kBlock), args, 0,
-                                            st));
+        D2IC_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kBlock), args, 0, st));
     }
     D2IC_HIP(hipEventRecord(lead->ev_end, st));
     lead->timed = true;
