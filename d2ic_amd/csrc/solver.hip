@@ -1012,6 +1012,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                                                           Workspace* __restrict__ ws, int max_steps, ServeHost* sh) {
     static_assert(!(kServe && kSingle), "a served session has many slots");
     if constexpr (kSingle) P = 1;
+    if constexpr (!kServe) sh = nullptr;  // fixed batches have no mailbox: fold the served paths away
     // served sessions read descriptors from an LDS copy refreshed on admission
     __shared__ ProbDesc s_desc[kServe ? kMaxSlots : 1];
     __shared__ int s_flags[kMaxProblems];
