@@ -499,7 +499,7 @@ __device__ __forceinline__ float group_sum(float v) {
 // Block-level sum of two per-lane values in fixed order: wave sums through
 // DPP, then the 16 wave partials summed across the first row of wave 0
 // (quad_perm, row_half_mirror, row_mirror); lane 0 stores them.
-static_assert(kWavesPerBlock == 16, "block_store2 folds one 16-lane row");
+static_assert(kWavesPerBlock <= 16, "block_store2 folds one 16-lane row");
 __device__ __forceinline__ void block_store2(double a, double b, double* s_red, double* out) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     a = wave_sum63(a);
