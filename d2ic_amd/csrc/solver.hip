@@ -253,8 +253,8 @@ constexpr int kNodesPerWave = kWave / kLanesPerNode;
 constexpr int kNodesPerItem = kNodesPerWave * kWavesPerBlock;  // node item = one block
 constexpr int kBoxMax = 512;  // floats of staged image per wave (8 loads per lane)
 
-struct TilePix {  // per-lane pixel data of one chunk (f is loaded at use)
-    float x, y;
+struct TilePix {  // per-lane pixel data of one chunk (loaded one chunk ahead)
+    float x, y, f;
     float4 N;
 };
 
@@ -292,6 +292,7 @@ __device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int 
     const int i = c * kWave + lane;
     r.x = gld(D->px + t0)[i];
     r.y = gld(D->py + t0)[i];
+    r.f = gld(D->F + t0)[i];
     const f4v nv = gld(reinterpret_cast<const f4v*>(D->N4 + t0))[i];
     r.N = make_float4(nv.x, nv.y, nv.z, nv.w);
     return r;
@@ -416,11 +417,11 @@ __device__ __forceinline__ void box_store(const TileBox& g, int lane, const floa
 // accumulated per lane into r^2 (FP64) and the 8 gradient partials
 // acc[2a + {0,1}] += r * (dg/dx, dg/dy) * N_a.  Samples inside the staged box
 // read LDS; others read global memory (same values).  Dead lanes add zeros.
-__device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, const float* Ft, int i, const TilePix& px,
+__device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, const TilePix& px,
                                             const float2 (&uc)[4], const TileBox& g, bool live,
                                             const __attribute__((address_space(3))) float* box, float (&acc)[8],
                                             double& r2) {
-    const float f = gld(Ft)[i];
+    const float f = px.f;
     const float4 N = px.N;
     const float ux = N.x * uc[0].x + N.y * uc[1].x + N.z * uc[2].x + N.w * uc[3].x;
     const float uy = N.x * uc[0].y + N.y * uc[1].y + N.z * uc[2].y + N.w * uc[3].y;
@@ -1162,7 +1163,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                             pn = tile_pix(Da, ta, c + 1, lane);
                         else if (it + 1 < t1)
                             pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
-                        chunk_accum(Da, Da->F + (size_t)ta * kTile, c * kWave + lane, px, uc, b_cur,
+                        chunk_accum(Da, px, uc, b_cur,
                                     c * kWave + lane < b_cur.cnt, my_box, acc, r2);
                         px = pn;
                     }
