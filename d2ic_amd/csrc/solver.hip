@@ -529,12 +529,45 @@ __device__ __forceinline__ void block_store2(double a, double b, double* s_red, 
     __syncthreads();
 }
 
+// Static topology of one node item as every lane of the block reads it
+// (node n of lane, slot sub): first spring neighbour and weight, the node's
+// CSR range and first tile entry.  Single-problem solves keep their block's
+// item here for the whole launch, so node items start with their fresh loads.
+struct NodeCache {
+    int j[kBlock];
+    float w[kBlock];
+    int e0[kBlock], e1[kBlock], ent[kBlock];
+};
+
+__device__ __forceinline__ void node_cache_fill(const ProbDesc* __restrict__ D, int item, NodeCache* nc) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int sub = lane % kLanesPerNode;
+    const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
+    int j = -1, e0 = 0, e1 = 0, ent = -1;
+    float w = 0.f;
+    if (n < D->nnodes) {
+        const int md = D->nb_maxdeg;
+        if (sub < md) {
+            j = gld(D->nb_idx)[(size_t)n * md + sub];
+            w = gld(D->nb_wp)[(size_t)n * md + sub];
+        }
+        e0 = gld(D->node_ptr)[n];
+        e1 = gld(D->node_ptr)[n + 1];
+        if (e0 + sub < e1) ent = gld(D->node_ent)[e0 + sub];
+    }
+    nc->j[threadIdx.x] = j;
+    nc->w[threadIdx.x] = w;
+    nc->e0[threadIdx.x] = e0;
+    nc->e1[threadIdx.x] = e1;
+    nc->ent[threadIdx.x] = ent;
+}
+
 // A-step node item (128 nodes, 8 lanes per node): materialise d, g.d and the
 // spring energy at the candidate (both halves of every ordered pair).  Loads
 // go out in two rounds: the node's own values with the neighbour list, then
 // the neighbours' values (further neighbours, past 8, in later rounds).
 __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
-                            double* s_red, int par) {
+                            double* s_red, int par, const NodeCache* nc) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
     const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -548,7 +581,10 @@ __device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags,
         const int md = D->nb_maxdeg;
         int j = -1;
         float w = 0.f;
-        if (sub < md) {
+        if (nc) {
+            j = nc->j[threadIdx.x];
+            w = nc->w[threadIdx.x];
+        } else if (sub < md) {
             j = gld(D->nb_idx)[(size_t)n * md + sub];
             w = gld(D->nb_wp)[(size_t)n * md + sub];
         }
@@ -606,7 +642,7 @@ __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
 // entry and neighbour values; the entry's tile partials.  Entries and
 // neighbours past the first 8 follow in order.
 __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, double* s_red,
-                            int par) {
+                            int par, const NodeCache* nc) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int sub = lane % kLanesPerNode;
@@ -620,18 +656,26 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
     const float* Dn = D->D[dcur ^ 1];
     if (valid) {
         const int md = D->nb_maxdeg;
-        const int e0 = gld(D->node_ptr)[n];
-        const int e1 = gld(D->node_ptr)[n + 1];
-        int j = -1;
+        int e0, e1, j = -1, ent;
         float w = 0.f;
-        if (sub < md) {
-            j = gld(D->nb_idx)[(size_t)n * md + sub];
-            w = gld(D->nb_wp)[(size_t)n * md + sub];
+        if (nc) {
+            e0 = nc->e0[threadIdx.x];
+            e1 = nc->e1[threadIdx.x];
+            j = nc->j[threadIdx.x];
+            w = nc->w[threadIdx.x];
+            ent = nc->ent[threadIdx.x];
+        } else {
+            e0 = gld(D->node_ptr)[n];
+            e1 = gld(D->node_ptr)[n + 1];
+            if (sub < md) {
+                j = gld(D->nb_idx)[(size_t)n * md + sub];
+                w = gld(D->nb_wp)[(size_t)n * md + sub];
+            }
+            ent = e0 + sub < e1 ? gld(D->node_ent)[e0 + sub] : -1;
         }
         const float2 dn = ld2(Dn, n);
         const float2 u0 = ld2(U, n);
         go = ld2(D->G[cur], n);
-        const int ent = e0 + sub < e1 ? gld(D->node_ent)[e0 + sub] : -1;
         float2 uj = make_float2(0.f, 0.f), dj = uj;
         if (j >= 0) {
             uj = ld2(U, j);
@@ -1031,6 +1075,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ int s_stop;
     __shared__ double s_red[2 * kWavesPerBlock];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
+    __shared__ typename std::conditional<kSingle, NodeCache, int>::type s_nc_store;
+    NodeCache* s_nc = kSingle ? reinterpret_cast<NodeCache*>(&s_nc_store) : nullptr;
     __attribute__((address_space(3))) float* my_box = (__attribute__((address_space(3))) float*)s_box[threadIdx.x / kWave];
 
     const int lane = threadIdx.x & (kWave - 1);
@@ -1068,6 +1114,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         s_hdr[1] = ws->n_chunks_active;
         s_hdr[2] = ws->any_active;
     }
+    // single problem: the block's node item is the same in every step that has
+    // node work (items [ceil(vb nC / nb), ...) with nC = nchunks); cache its
+    // static topology when the block owns exactly one item
+    int nc_item = -1;
+    if constexpr (kSingle) {
+        const int nch = descs[0].nchunks;
+        const int c0 = item_begin(vb, nch, nb);
+        if (item_begin(vb + 1, nch, nb) == c0 + 1) {
+            nc_item = c0;
+            node_cache_fill(descs, c0, s_nc);
+        }
+    }
     __syncthreads();
 
     unsigned long long seen_bell = ~0ull;  // served mode, master wave: last doorbell scanned
@@ -1089,9 +1147,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             if (fl & F_COPY)
                 node_item_copy(D, c - s_coff[p]);
             else if (isA)
-                node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par);
+                node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par, c == nc_item ? s_nc : nullptr);
             else
-                node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red, par);
+                node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red, par, c == nc_item ? s_nc : nullptr);
         }
 
         // ---- tiles (A only): contiguous share per wave, software-pipelined:
