@@ -1,28 +1,32 @@
 // Global FE-DIC solver: NLCG (Polak-Ribiere+, Armijo backtracking) on
 //   J(u) = 1/2 mean_p (g(x_p + u(x_p)) - f(x_p))^2 + alpha_reg * E_spring(u)
 // (reference: d2ic/solver_global_cg.py:205-469), as ONE persistent cooperative
-// kernel that runs every iteration of every problem of a batch.
+// kernel that runs every iteration of every problem of a batch (or of a served
+// session).
 //
 // Layout (built once per PixelAssets by d2ic_cg_plan_create):
-//   * pixels grouped by element and cut into tiles of <= 64 pixels; per pixel
-//     structure-of-arrays x, y, N0..N3 (float4) and the pre-sampled f(x_p);
-//     per tile its pixel range and its 4 node ids;
+//   * pixels grouped by element and cut into tiles of <= 256 pixels (4 chunks
+//     of 64); per pixel structure-of-arrays x, y, N0..N3 (float4) and the
+//     pre-sampled f(x_p); per tile its pixel count, 4 node ids and static
+//     pixel bounding box;
 //   * a node -> (tile, local node) gather table, so the image part of the
 //     gradient is assembled deterministically (no float atomics);
 //   * the spring stencil with the combined weight w_ij * (c_i + c_j) / 2.
 //
-// Each iteration is two grid-wide steps separated by a tree barrier whose last
-// arriving block (the root) reduces the step, decides every problem and
-// publishes the plan of the next step:
+// Each iteration is two grid-wide steps separated by a flag barrier (every
+// block publishes its arrival; the first wave of block 0 waits for all of them
+// and writes every block's release word).  After each barrier every block
+// reduces the finished step from the same memory in the same order and runs
+// the same state machine, so the next step's plan is identical everywhere:
 //   A-step: every tile evaluates the residual at the candidate
 //           u + alpha * d (d formed on the fly from g, d_prev and beta) and
 //           writes its partial gradient (8 floats) and its sum of r^2; node
-//           chunks materialise d, and accumulate g.d and the spring energy at
+//           items materialise d, and accumulate g.d and the spring energy at
 //           the candidate.
-//   B-step: the Armijo decision is taken (redundantly and identically in
-//           every block); if the trial is accepted, node chunks assemble the
-//           gradient at the new point through the gather table and reduce
-//           |g|^2 and g.(g - g_prev) for the next beta and the stop test.
+//   B-step: the Armijo decision has been taken; if the trial is accepted,
+//           node items assemble the gradient at the new point through the
+//           gather table and reduce |g|^2 and g.(g - g_prev) for the next
+//           beta and the stop test.
 // The accepted trial's A-step is also the next iteration's value+gradient
 // evaluation, so one pixel pass per iteration is enough when alpha = 1 is
 // accepted (the reference's usual case), without changing the iterates.
@@ -30,9 +34,9 @@
 // Reductions are deterministic and independent of the batch composition:
 // per-tile sums of r^2 are converted exactly to a two-word fixed-point integer
 // (resolution 2^-40), pre-summed per block in LDS and added with integer
-// atomics spread over 16 slots; node-chunk partials are stored per chunk and
-// summed in fixed order by the root.  Dot products and the objective are
-// accumulated in FP64 over FP32 arithmetic.
+// atomics spread over 16 slots; node-item partials are stored per item and
+// summed in a fixed order.  Dot products and the objective are accumulated in
+// FP64 over FP32 arithmetic.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
