@@ -728,7 +728,7 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
     block_store2(gg, gnum, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
 }
 
-// State transition of one problem after a step (root block only).  just_did_a:
+// State transition of one problem after a step (run identically in every block).  just_did_a:
 // the finished step was an A-step (Armijo decision follows), else a B-step
 // (stop test and beta follow).  fin: node-chunk sums of the finished step.
 __device__ int decide(const DecideDesc& D, ProbState& st, bool just_did_a, double sr2, bool bad, double fin0,
