@@ -31,6 +31,7 @@ from .propagator_previous import PreviousDisplacementPropagator
 from .propagator_constant_velocity import ConstantVelocityPropagator
 from .solver_translation_zncc import TranslationZNCCSolver
 from .solver_global_cg import GlobalCGSolver
+from .solver_local_gn import LocalGaussNewtonSolver, LocalGNResult
 
 __all__ = [
     "Array",
@@ -65,4 +66,6 @@ __all__ = [
     "ConstantVelocityPropagator",
     "TranslationZNCCSolver",
     "GlobalCGSolver",
+    "LocalGaussNewtonSolver",
+    "LocalGNResult",
 ]

@@ -26,6 +26,7 @@ SOURCES = {
     "solver.hip": [],
     "post.hip": [],
     "adjoint.hip": [],
+    "local_gn.hip": [],
 }
 
 COMMON = [

@@ -191,6 +191,25 @@ int d2ic_adj_diag_inv(const d2ic_cg_plan* plan, const float* K, float alpha_reg,
 int d2ic_adj_spring(const d2ic_cg_plan* plan, const float* v, int32_t nv, float* out, float* work, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Local Gauss-Newton nodal refinement (replaces LocalGaussNewtonSolver's
+ * _local_sweeps, solver_local_gn.py:205-309, with jacobi_nodal_step_spring,
+ * :312-414, and the central-difference gradient images, :195-202).
+ * Arrays are the PixelAssets fields in the reference layout (device):
+ * pixel_coords (np,2) f32, pixel_nodes (np,4) i32, pixel_shapeN (np,4) f32,
+ * node_pixel_index / node_N_weight (nn,max_deg_pix), node_pixel_degree (nn),
+ * node_neighbor_index / weight (nn,max_deg_nb), degree (nn), node_reg_weight
+ * (nn).  u (nn,2) holds u0 and receives the result after n_sweeps sweeps;
+ * hist (n_sweeps,2) = (RMS residual, RMS update) per sweep, or NULL.
+ * interp: 0 cubic, 1 linear.
+ * ------------------------------------------------------------------------ */
+int d2ic_lgn_solve(const float* pixel_coords, const int32_t* pixel_nodes, const float* pixel_shapeN, int64_t np,
+                   const int32_t* node_pixel_index, const float* node_N_weight, const int32_t* node_pixel_degree,
+                   int64_t max_deg_pix, const int32_t* nb_idx, const int32_t* nb_deg, const float* nb_w,
+                   const float* node_reg_weight, int64_t max_deg_nb, int64_t nn, const float* ref_image,
+                   const float* def_image, int32_t H, int32_t W, int32_t interp, float* u, int32_t n_sweeps, float lam,
+                   float max_step, float alpha_reg, float omega, float* hist, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Translation ZNCC (replaces _match_centers_jax, solver_translation_zncc.py:118-168)
  * ------------------------------------------------------------------------ */
 /* centers (n,2) f32; ref (Hr,Wr) and def (Hd,Wd) f32 row-major.  u_out (n,2)

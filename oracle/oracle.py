@@ -41,6 +41,14 @@ What pins each function (tests named in brackets):
 * ``idw`` -- closed form on tiny inputs and a numpy restatement [test_oracle_strain_idw].
 * ``strain`` -- the reference's closed form (``tests/test_strain_smoke.py:10-40``)
   and exact E for affine fields [test_oracle_strain_idw].
+* ``image_gradient`` -- exact on ramps, zero edges [test_oracle_local_gn].
+  ``local_gn_step`` -- closed forms: one step recovers a displacement when the
+  residual is linear in it, lam shrinks an isotropic step by 1/(1 + 2 lam),
+  max_step clips the norm, omega scales the update, a node with only springs
+  moves to the weighted mean of its neighbours, a node with neither pixels nor
+  springs stays [test_oracle_local_gn].  ``local_gn_sweeps`` -- recovers a
+  sub-pixel translation of synthetic speckle with a decreasing residual
+  [test_oracle_local_gn].
 """
 from __future__ import annotations
 
@@ -331,3 +339,97 @@ def strain(u, nodes, nb_idx, nb_deg, gauge_length=0.0, eps=1e-8, dtype=np.float3
     getattr(lib(), f"orc_strain_{s}")(_p(uu), _p(X), _i64(n), _p(bi), _p(bd), _i64(bi.shape[1]), R(gauge_length),
                                       R(eps), _p(F), _p(E), _p(V))
     return F, E, V
+
+
+# --------------------------------------------------------------------------
+# local Gauss-Newton refinement (numpy restatement over the C sampler)
+# --------------------------------------------------------------------------
+
+def image_gradient(im, dtype=np.float32):
+    """Central differences with zero edges (solver_local_gn.py:186-202)."""
+    im = np.asarray(im, dtype)
+    gx = np.zeros_like(im)
+    gy = np.zeros_like(im)
+    half = dtype(0.5)
+    gx[:, 1:-1] = half * (im[:, 2:] - im[:, :-2])
+    gy[1:-1, :] = half * (im[2:, :] - im[:-2, :])
+    return gx, gy
+
+
+def local_gn_step(u, r, gx, gy, assets, lam=0.1, max_step=0.2, alpha_reg=0.0, omega=0.5, dtype=np.float32):
+    """jacobi_nodal_step_spring (solver_local_gn.py:312-414) for per-pixel r and
+    sampled gradient images gx, gy; returns the relaxed update of u (Nn, 2)."""
+    R = dtype
+    u = np.asarray(u, R).reshape(-1, 2)
+    r = np.asarray(r, R)
+    gx = np.asarray(gx, R)
+    gy = np.asarray(gy, R)
+    npi = np.asarray(assets["node_pixel_index"])
+    npw = np.asarray(assets["node_N_weight"], R)
+    deg_pix = np.asarray(assets["node_pixel_degree"]).reshape(-1)
+    nbi = np.asarray(assets["node_neighbor_index"])
+    nbw = np.asarray(assets["node_neighbor_weight"], R)
+    deg_nb = np.asarray(assets["node_neighbor_degree"]).reshape(-1)
+    nrw = np.asarray(assets["node_reg_weight"], R).reshape(-1)
+    nn = u.shape[0]
+    mdp = npi.shape[1] if npi.ndim == 2 else 0
+    mdn = nbi.shape[1] if nbi.ndim == 2 else 0
+    pm = np.arange(mdp)[None, :] < deg_pix[:, None]
+    idx = np.where(pm, npi, 0) if mdp else np.zeros((nn, 0), np.int64)
+    Ni = np.where(pm, npw, R(0)) if mdp else np.zeros((nn, 0), R)
+    Jx = gx[idx] * Ni
+    Jy = gy[idx] * Ni
+    ri = r[idx]
+    g_img = np.stack([(ri * Jx).sum(1), (ri * Jy).sum(1)], 1).astype(R)
+    H00 = (Jx * Jx).sum(1)
+    H01 = (Jx * Jy).sum(1)
+    H11 = (Jy * Jy).sum(1)
+    lam_loc = R(lam) * (H00 + H11 + R(1e-12))
+    nm = np.arange(mdn)[None, :] < deg_nb[:, None]
+    ids = np.where(nm, nbi, 0) if mdn else np.zeros((nn, 0), np.int64)
+    w = np.where(nm, nbw, R(0)) if mdn else np.zeros((nn, 0), R)
+    un = np.where(nm[..., None], u[ids], R(0)) if mdn else np.zeros((nn, 0, 2), R)
+    diff = u[:, None, :] - un
+    alpha_loc = R(alpha_reg) * nrw
+    g_reg = alpha_loc[:, None] * (w[..., None] * diff).sum(1)
+    sumw = w.sum(1)
+    H = np.zeros((nn, 2, 2), R)
+    H[:, 0, 0] = H00 + lam_loc + alpha_loc * sumw + R(1e-8)
+    H[:, 0, 1] = H01
+    H[:, 1, 0] = H01
+    H[:, 1, 1] = H11 + lam_loc + alpha_loc * sumw + R(1e-8)
+    g = (g_img + g_reg).astype(R)
+    delta = -np.linalg.solve(H, g[..., None])[..., 0].astype(R)
+    has = (deg_pix > 0) | ((R(alpha_reg) != 0) & (deg_nb > 0))
+    delta = np.where(has[:, None], delta, R(0))
+    nrm = np.sqrt((delta * delta).sum(1))
+    factor = np.minimum(R(1), R(max_step) / (nrm + R(1e-12)))
+    delta = delta * factor[:, None]
+    return (u + R(omega) * delta).astype(R)
+
+
+def local_gn_sweeps(u0, ref_image, def_image, assets, lam=1e-3, max_step=0.2, alpha_reg=0.0, omega=0.5, n_sweeps=3,
+                    interp="cubic", dtype=np.float32):
+    """_local_sweeps with _pixel_state (solver_local_gn.py:205-309): returns
+    (u, history (n_sweeps, 2) = [RMS residual, RMS update])."""
+    R = dtype
+    gx2, gy2 = image_gradient(def_image, R)
+    coords = np.asarray(assets["pixel_coords_ref"], R).reshape(-1, 2)
+    pn = np.asarray(assets["pixel_nodes"]).reshape(-1, 4)
+    N = np.asarray(assets["pixel_shapeN"], R).reshape(-1, 4)
+    I1 = sample(ref_image, coords, interp, dtype=R)
+    u = np.asarray(u0, R).reshape(-1, 2).copy()
+    hist = np.full((int(n_sweeps), 2), np.nan, R)
+    for k in range(int(n_sweeps)):
+        upix = (N[..., None] * u[pn]).sum(1)
+        xd = (coords + upix).astype(R)
+        r = (sample(def_image, xd, interp, dtype=R) - I1).astype(R)
+        gxd = sample(gx2, xd, interp, dtype=R)
+        gyd = sample(gy2, xd, interp, dtype=R)
+        un = local_gn_step(u, r, gxd, gyd, assets, lam=lam, max_step=max_step, alpha_reg=alpha_reg, omega=omega,
+                           dtype=R)
+        step = un - u
+        hist[k, 0] = np.sqrt(np.mean(r.astype(np.float64) ** 2))
+        hist[k, 1] = np.sqrt(np.mean(step.astype(np.float64) ** 2))
+        u = un
+    return u, hist
