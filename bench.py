@@ -373,6 +373,47 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
     return res
 
 
+def local_refine_leg(dic, ref, frame, u0, n_pixels, sweeps=3, reps=5):
+    """LocalGaussNewtonSolver on the headline workload (the reference
+    tutorial's refinement: 3 sweeps, lam 0.1, max_step 0.2, omega 0.5) from the
+    global solution.  Algorithmic bytes per sweep: pixel pass 56 B/px (coords,
+    4 node ids, 4 shape values, f, and r, dg/dx, dg/dy written) + node pass
+    20 B per node->pixel entry (index, weight, gathered r, dg/dx, dg/dy) with
+    4 entries per pixel."""
+    import torch
+
+    from d2ic_amd import DICMeshBased, LocalGaussNewtonSolver, MeshDICConfig
+
+    w = WORKLOAD
+    solver = LocalGaussNewtonSolver(lam=0.1, max_step=0.2, omega=0.5, interpolation=w["interp"])
+    loc = DICMeshBased(dic.mesh, solver, MeshDICConfig(max_iters=sweeps, tol=w["tol"], reg_strength=w["reg"],
+                                                       save_history=True))
+    loc.prepare(ref, dic.state.assets)
+    loc.set_initial_guess(u0)
+    sol = solver.solve(loc.state, frame)  # warm-up
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        sol = solver.solve(loc.state, frame)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    hist = sol.history.cpu().numpy()
+    step = float((sol.u_nodal - u0).abs().max()) if u0 is not None else None
+    per_sweep = float(np.median(ms)) / sweeps
+    b = (56 + 4 * 20) * n_pixels
+    res = dict(sweeps=sweeps, ms_per_refine=float(np.median(ms)), ms_per_sweep=per_sweep,
+               algorithmic_bytes_per_sweep=b, achieved_GBs=b / (per_sweep * 1e-3) / 1e9,
+               rms_residual_per_sweep=hist[:, 0].tolist(), rms_update_per_sweep=hist[:, 1].tolist(),
+               max_update_px=step)
+    log(f"[local] {sweeps} sweeps {res['ms_per_refine']:.2f} ms ({per_sweep:.2f} ms/sweep, "
+        f"{res['achieved_GBs']:.0f} GB/s algorithmic); RMS residual {hist[0, 0]:.3f} -> {hist[-1, 0]:.3f}")
+    return res
+
+
 def measured_traffic():
     """HBM bytes per k_cg_persistent launch on this workload, from the committed
     rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or None."""
@@ -417,6 +458,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the served-streams leg")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-local", action="store_true", help="skip the local Gauss-Newton refinement leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -490,6 +532,11 @@ def main() -> int:
     traffic = measured_traffic()
     log(f"[bench] {K} frames in {elapsed:.3f} s -> {fps:.2f} frames/s; iters {iters}; passes {passes}")
 
+    local = None
+    if not args.no_local:
+        log("[bench] local refinement leg: 3 Gauss-Newton sweeps from the last frame's solution")
+        local = local_refine_leg(dic, ref, frames[K - 1] if K > 0 else frames[0], u_prev, Np)
+
     batch = serving = None
     if not args.no_batch:
         log("[bench] batched leg: 64 uneven problems")
@@ -549,6 +596,7 @@ def main() -> int:
         "batch": batch,
         "serving": serving,
         "differentiation": diff,
+        "local_refine": local,
     }
     print(json.dumps(line), flush=True)
     # release device objects before interpreter teardown (profilers finalise then)
