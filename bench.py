@@ -414,6 +414,33 @@ def local_refine_leg(dic, ref, frame, u0, n_pixels, sweeps=3, reps=5):
     return res
 
 
+def discrepancy_leg(dic, ref, frame, u, n_pixels, reps=5):
+    """compute_discrepancy_map_ref on the headline workload at the solved u
+    (one map + RMS, the RMS read back to the host inside the timed call).
+    Algorithmic bytes: 44 B per pixel (coords, 4 node ids, 4 shape values,
+    image index) + both images read once + the map written (12 B per image
+    pixel)."""
+    import torch
+
+    from d2ic_amd.discrepancy import compute_discrepancy_map_ref
+
+    pd = dic.state.assets.pixel_data
+    H, W = (int(v) for v in pd.image_shape)
+    mp, rms = compute_discrepancy_map_ref(ref_image=ref, def_image=frame, u_nodal=u, pixel_assets=pd)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        mp, rms = compute_discrepancy_map_ref(ref_image=ref, def_image=frame, u_nodal=u, pixel_assets=pd)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    b = 44 * n_pixels + 12 * H * W
+    res = dict(ms_per_map=float(np.median(ms)), rms=float(rms), algorithmic_bytes=b,
+               achieved_GBs=b / (float(np.median(ms)) * 1e-3) / 1e9)
+    log(f"[discrepancy] map + RMS {res['ms_per_map']:.2f} ms ({res['achieved_GBs']:.0f} GB/s algorithmic), "
+        f"RMS {rms:.3f}")
+    return res
+
+
 def measured_traffic():
     """HBM bytes per k_cg_persistent launch on this workload, from the committed
     rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or None."""
@@ -536,6 +563,8 @@ def main() -> int:
     if not args.no_local:
         log("[bench] local refinement leg: 3 Gauss-Newton sweeps from the last frame's solution")
         local = local_refine_leg(dic, ref, frames[K - 1] if K > 0 else frames[0], u_prev, Np)
+        if u_prev is not None:
+            local["discrepancy"] = discrepancy_leg(dic, ref, frames[K - 1], u_prev, Np)
 
     batch = serving = None
     if not args.no_batch:

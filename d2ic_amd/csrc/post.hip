@@ -256,18 +256,31 @@ __global__ void k_strain(const float* __restrict__ u, const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Discrepancy map: bilinear_sample_jax (discrepancy.py:51-83) at x and x+u(x),
-// residual scattered to roi_mask_flat, NaN elsewhere; per-block FP64 partial
-// sums for the RMS.
+// Discrepancy map (discrepancy.py:86-127).  bilinear_sample_jax (:51-83) at x
+// and x + u(x) with NaN outside [0, H-1) x [0, W-1); the residual is scattered
+// like jnp.bincount(roi_flat, weights=r * valid) / max(bincount(valid), 1), NaN
+// where no valid pixel landed.  Every operation is a separately rounded FP32
+// operation in the reference's order (no contraction), so a pixel's residual
+// is the reference's FP32 value.  A NaN residual adds NaN to its image pixel
+// (the reference's NaN * 0 weight); per-block FP64 partials give the RMS.
+// (hipcc contracts a * b + c into an FMA by default, also inside the
+// __fmul_rn / __fadd_rn helpers, so these bodies switch contraction off and
+// use plain operators.)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float bilin_nan(const float* img, int H, int W, float x, float y) {
+#pragma clang fp contract(off)
     const float xc = x - 0.5f, yc = y - 0.5f;
     const int j0 = (int)floorf(xc), i0 = (int)floorf(yc);
     const float fx = xc - (float)j0, fy = yc - (float)i0;
     if (!(i0 >= 0 && i0 < H - 1 && j0 >= 0 && j0 < W - 1)) return NAN;
     const float v00 = img[(size_t)i0 * W + j0], v01 = img[(size_t)i0 * W + j0 + 1];
     const float v10 = img[(size_t)(i0 + 1) * W + j0], v11 = img[(size_t)(i0 + 1) * W + j0 + 1];
-    return v00 * (1.0f - fx) * (1.0f - fy) + v01 * fx * (1.0f - fy) + v10 * (1.0f - fx) * fy + v11 * fx * fy;
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    float v = v00 * gx * gy;
+    v = v + v01 * fx * gy;
+    v = v + v10 * gx * fy;
+    v = v + v11 * fx * fy;
+    return v;
 }
 
 __global__ void k_nan_fill(float* __restrict__ m, int64_t n) {
@@ -275,20 +288,48 @@ __global__ void k_nan_fill(float* __restrict__ m, int64_t n) {
     if (t < n) m[t] = NAN;
 }
 
+// zero sums and counts (a fill kernel rather than a memset node, so that a
+// caller may capture the launch sequence into a graph)
+__global__ void k_disc_clear(float* __restrict__ sum, int32_t* __restrict__ cnt, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        sum[t] = 0.f;
+        cnt[t] = 0;
+    }
+}
+
+// FP32 add by compare-and-swap: IEEE round-to-nearest with denormals and NaN
+// propagated, as a sequential bincount would (one CAS per pixel when the
+// image pixels are distinct, the case of every asset builder).
+__device__ __forceinline__ void cas_add(float* p, float v) {
+    unsigned* q = reinterpret_cast<unsigned*>(p);
+    unsigned old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (true) {
+        const unsigned nw = __float_as_uint(__uint_as_float(old) + v);
+        const unsigned seen = atomicCAS(q, old, nw);
+        if (seen == old) break;
+        old = seen;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_discrepancy(const float* __restrict__ ref, const float* __restrict__ def,
                                                      int H, int W, const float* __restrict__ u,
                                                      const float* __restrict__ xy, const int32_t* __restrict__ pn,
                                                      const float* __restrict__ N, const int32_t* __restrict__ rf,
-                                                     int64_t np, float* __restrict__ map, double* __restrict__ part) {
-    __shared__ double s0[256], s1[256];
+                                                     int64_t np, float* __restrict__ sum, int32_t* __restrict__ cnt,
+                                                     double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double s0[256], s1[256], s2[256];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0;
     if (p < np) {
+        // pixel displacement: sum_a N_a u_a in node order (np.sum over axis 1)
         float ux = 0.f, uy = 0.f;
         for (int k = 0; k < 4; ++k) {
             const int n = pn[4 * p + k];
-            ux += N[4 * p + k] * u[2 * n];
-            uy += N[4 * p + k] * u[2 * n + 1];
+            const float tx = N[4 * p + k] * u[2 * n], ty = N[4 * p + k] * u[2 * n + 1];
+            ux = k == 0 ? tx : ux + tx;
+            uy = k == 0 ? ty : uy + ty;
         }
         const float x = xy[2 * p], y = xy[2 * p + 1];
         const float ir = bilin_nan(ref, H, W, x, y);
@@ -296,26 +337,42 @@ __global__ __launch_bounds__(256) void k_discrepancy(const float* __restrict__ r
         const float r = id - ir;
         const bool valid = isfinite(r);
         const int64_t idx = rf[p];
-        if (idx >= 0 && idx < (int64_t)H * W) map[idx] = valid ? r : NAN;
+        if (idx >= 0 && idx < (int64_t)H * W) {
+            cas_add(sum + idx, valid ? r : NAN);
+            if (valid) atomicAdd(cnt + idx, 1);
+        }
         if (valid) {
             a = (double)r * (double)r;
             b = 1.0;
+        } else {
+            c = 1.0;
         }
     }
     s0[threadIdx.x] = a;
     s1[threadIdx.x] = b;
+    s2[threadIdx.x] = c;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
             s0[threadIdx.x] += s0[threadIdx.x + off];
             s1[threadIdx.x] += s1[threadIdx.x + off];
+            s2[threadIdx.x] += s2[threadIdx.x + off];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s0[0];
-        part[2 * blockIdx.x + 1] = s1[0];
+        part[3 * blockIdx.x] = s0[0];
+        part[3 * blockIdx.x + 1] = s1[0];
+        part[3 * blockIdx.x + 2] = s2[0];
     }
+}
+
+// map = sum / count where a valid pixel landed, NaN elsewhere
+__global__ void k_discrepancy_finish(float* __restrict__ map, const int32_t* __restrict__ cnt, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = cnt[t];
+    map[t] = c > 0 ? __fdiv_rn(map[t], (float)c) : NAN;
 }
 
 // ---------------------------------------------------------------------------
@@ -436,29 +493,38 @@ int d2ic_discrepancy_map(const float* ref_image, const float* def_image, int32_t
                          const int32_t* roi_flat, int64_t n_pixels, float* map_out, float* rms, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(H > 0 && W > 0, "image shape must be positive");
+    D2IC_CHECK_ARG(n_pixels >= 0, "n_pixels must be >= 0");
     int64_t hw = (int64_t)H * W;
-    k_nan_fill<<<ceil_div(hw, 256), 256, 0, st>>>(map_out, hw);
-    D2IC_LAUNCH_CHECK();
     if (n_pixels == 0) {
+        k_nan_fill<<<ceil_div(hw, 256), 256, 0, st>>>(map_out, hw);
+        D2IC_LAUNCH_CHECK();
         if (rms) *rms = 0.f;
         return D2IC_OK;
     }
     int64_t nb = ceil_div(n_pixels, 256);
-    Tmp part;
-    D2IC_HIP(part.alloc(sizeof(double) * 2 * nb, st));
+    Tmp part, cnt;
+    D2IC_HIP(part.alloc(sizeof(double) * 3 * nb, st));
+    D2IC_HIP(cnt.alloc(sizeof(int32_t) * hw, st));
+    k_disc_clear<<<ceil_div(hw, 256), 256, 0, st>>>(map_out, cnt.as<int32_t>(), hw);
+    D2IC_LAUNCH_CHECK();
     k_discrepancy<<<nb, 256, 0, st>>>(ref_image, def_image, H, W, u, pixel_coords, pixel_nodes, pixel_shapeN,
-                                      roi_flat, n_pixels, map_out, part.as<double>());
+                                      roi_flat, n_pixels, map_out, cnt.as<int32_t>(), part.as<double>());
+    D2IC_LAUNCH_CHECK();
+    k_discrepancy_finish<<<ceil_div(hw, 256), 256, 0, st>>>(map_out, cnt.as<int32_t>(), hw);
     D2IC_LAUNCH_CHECK();
     if (rms) {
-        std::vector<double> h(2 * nb);
-        D2IC_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st));
+        std::vector<double> h(3 * nb);
+        D2IC_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st));
         D2IC_HIP(hipStreamSynchronize(st));
-        double a = 0.0, b = 0.0;
+        double a = 0.0, b = 0.0, c = 0.0;
         for (int64_t k = 0; k < nb; ++k) {
-            a += h[2 * k];
-            b += h[2 * k + 1];
+            a += h[3 * k];
+            b += h[3 * k + 1];
+            c += h[3 * k + 2];
         }
-        *rms = (float)std::sqrt(a / std::max(b, 1.0));
+        // the reference sums (r * valid)^2 with NaN * 0 = NaN for an invalid
+        // pixel, so one sample outside the domain makes the RMS NaN
+        *rms = c > 0.0 ? NAN : (float)std::sqrt(a / std::max(b, 1.0));
     }
     return D2IC_OK;
 }

@@ -233,7 +233,10 @@ int d2ic_strain(const float* u, const float* nodes, int64_t n_nodes, const int32
 
 /* Reference-domain residual map (compute_discrepancy_map_ref,
  * discrepancy.py:86-127): bilinear with NaN outside, averaged per image
- * pixel.  map_out (H*W) f32; rms written to *rms (host; synchronises). */
+ * pixel (an invalid pixel makes its image pixel NaN; image pixels no valid
+ * pixel reached are NaN).  map_out (H*W) f32; rms written to *rms (host;
+ * synchronises): NaN when any pixel's sample is invalid, as the reference's
+ * sum of (r * valid)^2 is, and 0 for no pixels.  rms may be NULL. */
 int d2ic_discrepancy_map(const float* ref_image, const float* def_image, int32_t H, int32_t W, const float* u,
                          const float* pixel_coords, const int32_t* pixel_nodes, const float* pixel_shapeN,
                          const int32_t* roi_flat, int64_t n_pixels, float* map_out, float* rms, void* stream);

@@ -49,6 +49,12 @@ What pins each function (tests named in brackets):
   springs stays [test_oracle_local_gn].  ``local_gn_sweeps`` -- recovers a
   sub-pixel translation of synthetic speckle with a decreasing residual
   [test_oracle_local_gn].
+* ``bilinear_nan`` / ``discrepancy_map`` -- exact at pixel centres, exact on
+  affine images (residual = a ux + b uy for a constant u), the validity edge
+  (the last image row/column is outside, the sample just inside is not), NaN
+  RMS as soon as one sample is invalid, duplicated image pixels averaged and
+  poisoned by an invalid one, and no pixels (all NaN, RMS 0)
+  [test_oracle_discrepancy].
 """
 from __future__ import annotations
 
@@ -433,3 +439,66 @@ def local_gn_sweeps(u0, ref_image, def_image, assets, lam=1e-3, max_step=0.2, al
         hist[k, 1] = np.sqrt(np.mean(step.astype(np.float64) ** 2))
         u = un
     return u, hist
+
+
+# --------------------------------------------------------------------------
+# discrepancy map (numpy restatement, FP32 in the reference's operation order)
+# --------------------------------------------------------------------------
+
+def bilinear_nan(image, coords, dtype=np.float32):
+    """bilinear_sample_jax (discrepancy.py:51-83): samples at (x, y) - 0.5,
+    NaN unless 0 <= floor(y - .5) < H - 1 and 0 <= floor(x - .5) < W - 1."""
+    R = dtype
+    image = np.asarray(image, R)
+    coords = np.asarray(coords, R).reshape(-1, 2)
+    H, W = image.shape
+    xc = coords[:, 0] - R(0.5)
+    yc = coords[:, 1] - R(0.5)
+    j0 = np.floor(xc).astype(np.int64)
+    i0 = np.floor(yc).astype(np.int64)
+    fx = xc - j0.astype(R)
+    fy = yc - i0.astype(R)
+    valid = (i0 >= 0) & (i0 < H - 1) & (j0 >= 0) & (j0 < W - 1)
+    i0v = np.clip(i0, 0, max(H - 2, 0))
+    j0v = np.clip(j0, 0, max(W - 2, 0))
+    v00 = image[i0v, j0v]
+    v01 = image[i0v, j0v + 1]
+    v10 = image[i0v + 1, j0v]
+    v11 = image[i0v + 1, j0v + 1]
+    one = R(1.0)
+    vals = v00 * (one - fx) * (one - fy) + v01 * fx * (one - fy) + v10 * (one - fx) * fy + v11 * fx * fy
+    return np.where(valid, vals, R(np.nan)).astype(R)
+
+
+def discrepancy_map(ref_image, def_image, u, assets, dtype=np.float32):
+    """compute_discrepancy_map_ref (discrepancy.py:86-127): returns
+    (map (H, W), rms).  r = I_def(x + u(x)) - I_ref(x); the map is
+    bincount(roi_flat, r * valid) / max(bincount(valid), 1), NaN where no
+    valid pixel landed; rms = sqrt(sum((r * valid)^2) / max(sum(valid), 1)),
+    which is NaN once any r is NaN (NaN * 0)."""
+    R = dtype
+    H, W = (int(v) for v in assets["image_shape"])
+    coords = np.asarray(assets["pixel_coords_ref"], R).reshape(-1, 2)
+    pn = np.asarray(assets["pixel_nodes"]).reshape(-1, 4)
+    N = np.asarray(assets["pixel_shapeN"], R).reshape(-1, 4)
+    rf = np.asarray(assets["roi_mask_flat"]).reshape(-1).astype(np.int64)
+    u = np.asarray(u, R).reshape(-1, 2)
+    prod = N[..., None] * u[pn]
+    disp = prod[:, 0]
+    for k in range(1, 4):
+        disp = disp + prod[:, k]
+    i_ref = bilinear_nan(ref_image, coords, R)
+    i_def = bilinear_nan(def_image, coords + disp, R)
+    with np.errstate(invalid="ignore"):
+        res = (i_def - i_ref).astype(R)
+        valid = np.isfinite(res)
+        vf = valid.astype(R)
+        w = res * vf
+        denom = max(float(vf.sum(dtype=np.float64)), 1.0)
+        rms = float(np.sqrt((w.astype(np.float64) ** 2).sum() / denom))
+        acc = np.zeros(H * W, R)
+        cnt = np.zeros(H * W, R)
+        np.add.at(acc, rf, w)
+        np.add.at(cnt, rf, vf)
+        out = np.where(cnt > 0, acc / np.maximum(cnt, R(1)), R(np.nan)).astype(R)
+    return out.reshape(H, W), rms
