@@ -1515,6 +1515,41 @@ __global__ void k_heads(const unsigned long long* __restrict__ k, int64_t n, uin
     if (s < n) head[s] = (s == 0 || k[s] != k[s - 1]) ? 1 : 0;
 }
 
+// Elements wider than kStripMinW pixels are split into kStripW-wide column
+// strips (relative to the element's first pixel column): a 256-pixel tile of
+// a strip is then a 16 x 16 block whose staged image box (21 x 21 + the
+// displacement range) fits the wave's LDS slot, where a row-major tile of a
+// 32-pixel-wide element (8 rows x 32 columns, 37-38 x 13-14 box) often does
+// not.  One thread per element group; key = (group << 16) | strip, in the
+// group's current (row-major) order, so a stable sort keeps rows in order.
+constexpr int kStripW = 16;
+constexpr int kStripMinW = 24;
+__global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, int64_t np,
+                               const int32_t* __restrict__ perm, const float* __restrict__ coords,
+                               unsigned long long* __restrict__ key2, int* __restrict__ any_split) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int64_t s0 = gstart[g], s1 = (g + 1 < ng) ? gstart[g + 1] : np;
+    int xmin = 0, xmax = 0;
+    bool split = false;
+    if (s1 - s0 > kTile) {
+        xmin = INT32_MAX;
+        xmax = INT32_MIN;
+        for (int64_t s = s0; s < s1; ++s) {
+            const int c = (int)floorf(coords[2 * (int64_t)perm[s]]);
+            xmin = min(xmin, c);
+            xmax = max(xmax, c);
+        }
+        split = xmax - xmin + 1 > kStripMinW;
+    }
+    for (int64_t s = s0; s < s1; ++s) {
+        unsigned strip = 0;
+        if (split) strip = (unsigned)(((int)floorf(coords[2 * (int64_t)perm[s]]) - xmin) / kStripW);
+        key2[s] = ((unsigned long long)g << 16) | strip;
+    }
+    if (split) atomicOr(any_split, 1);
+}
+
 __global__ void k_group_ntiles(const int32_t* __restrict__ gstart, int64_t ng, int64_t np, int32_t* __restrict__ nt) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
@@ -1872,6 +1907,32 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         int32_t ng = 0;
         PH(hipMemcpyAsync(&ng, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         PH(hipStreamSynchronize(st));
+        {
+            // wide elements: regroup by (element, 16-pixel column strip)
+            Tmp any;
+            PH(any.alloc(sizeof(int), st));
+            PH(hipMemsetAsync(any.p, 0, sizeof(int), st));
+            k_group_strips<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), ng, n_pixels, perm.as<int32_t>(),
+                                                            pixel_coords, keys.as<unsigned long long>(),
+                                                            any.as<int>());
+            PH(hipGetLastError());
+            int hany = 0;
+            PH(hipMemcpyAsync(&hany, any.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            PH(hipStreamSynchronize(st));
+            if (hany) {
+                PH(hipMemcpyAsync(vals.p, perm.p, sizeof(int32_t) * n_pixels, hipMemcpyDeviceToDevice, st));
+                PH(hipcub::DeviceRadixSort::SortPairs(tmp.p, b1, keys.as<unsigned long long>(),
+                                                      keys_s.as<unsigned long long>(), vals.as<int32_t>(),
+                                                      perm.as<int32_t>(), (int)n_pixels, 0, 64, st));
+                k_heads<<<ceil_div(n_pixels, tb), tb, 0, st>>>(keys_s.as<unsigned long long>(), n_pixels,
+                                                               head.as<uint8_t>());
+                PH(hipGetLastError());
+                PH(hipcub::DeviceSelect::Flagged(tmp.p, b2, cnt_it, head.as<uint8_t>(), gstart.as<int32_t>(),
+                                                 nsel.as<int32_t>(), (int)n_pixels, st));
+                PH(hipMemcpyAsync(&ng, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                PH(hipStreamSynchronize(st));
+            }
+        }
         PH(ntg.alloc(sizeof(int32_t) * (ng + 1), st));
         PH(tstart.alloc(sizeof(int32_t) * (ng + 1), st));
         k_group_ntiles<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), ng, n_pixels, ntg.as<int32_t>());
