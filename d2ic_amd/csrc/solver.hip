@@ -1515,37 +1515,50 @@ __global__ void k_heads(const unsigned long long* __restrict__ k, int64_t n, uin
     if (s < n) head[s] = (s == 0 || k[s] != k[s - 1]) ? 1 : 0;
 }
 
-// Elements wider than kStripMinW pixels are split into kStripW-wide column
-// strips (relative to the element's first pixel column): a 256-pixel tile of
-// a strip is then a 16 x 16 block whose staged image box (21 x 21 + the
-// displacement range) fits the wave's LDS slot, where a row-major tile of a
-// 32-pixel-wide element (8 rows x 32 columns, 37-38 x 13-14 box) often does
-// not.  One thread per element group; key = (group << 16) | strip, in the
-// group's current (row-major) order, so a stable sort keeps rows in order.
-constexpr int kStripW = 16;
-constexpr int kStripMinW = 24;
+// Elements of more than 256 pixels are regrouped before tiling: ns =
+// ceil(w / 16) column strips of balanced width sw = ceil(w / ns), each cut
+// into blocks of floor(256 / sw) whole rows, so a tile is one block (<= 256
+// pixels, <= 16 columns).  Its staged image box, (sw + 5) x (rows + 5) plus
+// the displacement range, then fits the wave's 512-float LDS slot for every
+// element shape; row-major 256-pixel runs of a 21..32-pixel-wide element
+// (e.g. 8 rows x 32 columns, a 37-38 x 13-14 box) often do not, and their
+// samples fall back to global memory.  Elements of <= 256 pixels (the 16 px
+// elements of the benchmark) keep one tile.  One thread per element group;
+// key = (group << 20) | (strip << 10) | block over the group's current
+// (row-major) order, so a stable sort keeps the rows of a block in order.
 __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, int64_t np,
                                const int32_t* __restrict__ perm, const float* __restrict__ coords,
                                unsigned long long* __restrict__ key2, int* __restrict__ any_split) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int64_t s0 = gstart[g], s1 = (g + 1 < ng) ? gstart[g + 1] : np;
-    int xmin = 0, xmax = 0;
-    bool split = false;
-    if (s1 - s0 > kTile) {
-        xmin = INT32_MAX;
-        xmax = INT32_MIN;
+    int xmin = 0, ymin = 0, sw = 1, rpt = 1;
+    const bool split = s1 - s0 > kTile;
+    if (split) {
+        int xmax = INT32_MIN;
+        xmin = ymin = INT32_MAX;
         for (int64_t s = s0; s < s1; ++s) {
-            const int c = (int)floorf(coords[2 * (int64_t)perm[s]]);
+            const int64_t p = perm[s];
+            const int c = (int)floorf(coords[2 * p]), r = (int)floorf(coords[2 * p + 1]);
             xmin = min(xmin, c);
             xmax = max(xmax, c);
+            ymin = min(ymin, r);
         }
-        split = xmax - xmin + 1 > kStripMinW;
+        const int w = xmax - xmin + 1;
+        const int ns = (w + 15) / 16;
+        sw = (w + ns - 1) / ns;
+        rpt = kTile / sw;
     }
     for (int64_t s = s0; s < s1; ++s) {
-        unsigned strip = 0;
-        if (split) strip = (unsigned)(((int)floorf(coords[2 * (int64_t)perm[s]]) - xmin) / kStripW);
-        key2[s] = ((unsigned long long)g << 16) | strip;
+        unsigned long long sub = 0ull;
+        if (split) {
+            const int64_t p = perm[s];
+            const int c = (int)floorf(coords[2 * p]), r = (int)floorf(coords[2 * p + 1]);
+            const unsigned strip = (unsigned)min((c - xmin) / sw, 1023);
+            const unsigned blk = (unsigned)min((r - ymin) / rpt, 1023);
+            sub = ((unsigned long long)strip << 10) | blk;
+        }
+        key2[s] = ((unsigned long long)g << 20) | sub;
     }
     if (split) atomicOr(any_split, 1);
 }
@@ -1908,7 +1921,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PH(hipMemcpyAsync(&ng, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         PH(hipStreamSynchronize(st));
         {
-            // wide elements: regroup by (element, 16-pixel column strip)
+            // elements of more than one tile: regroup by (element, strip, row block)
             Tmp any;
             PH(any.alloc(sizeof(int), st));
             PH(hipMemsetAsync(any.p, 0, sizeof(int), st));
