@@ -318,6 +318,7 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
     act = plan.node_active
     win = torch.tensor((np.abs(X[:, 0] - W / 2) <= 100) & (np.abs(X[:, 1] - H / 2) <= 100), device="cuda") & act
     u_target = torch.tensor(np.stack([ux, uy], 1), dtype=torch.float32, device="cuda")
+    act_w, win_w = act.to(torch.float32), win.to(torch.float32)
 
     def forward_backward():
         f = ref.clone().requires_grad_(True)
@@ -329,7 +330,8 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
         torch.cuda.synchronize()
         t_fwd = time.perf_counter() - t0
         E = green_lagrange_torch(u, X, nb_idx, nb_deg)
-        loss = ((u - u_target)[act] ** 2).sum() + E[win, 0, 0].mean()
+        # masks as weights (boolean indexing would add torch's serial index backward)
+        loss = (((u - u_target) ** 2) * act_w[:, None]).sum() + (E[:, 0, 0] * win_w).sum() / win_w.sum()
         t0 = time.perf_counter()
         loss.backward()
         torch.cuda.synchronize()

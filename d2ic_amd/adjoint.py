@@ -82,11 +82,19 @@ class Linearization:
         return self._pc
 
     def solve(self, B, alpha, rtol: float = 1e-6, max_iter: int = 2000, check_every: int = 10,
-              precondition: bool = True):
+              precondition: bool = True, graph: bool = True, graph_after: int = 3):
         """Conjugate gradients on H X = B for every right-hand side of B (nv, n_nodes, 2),
         block-Jacobi preconditioned unless ``precondition=False``; FP64 dot
         products, per-RHS step sizes; stops when every |R| <= rtol |B|
-        (tested every ``check_every`` iterations)."""
+        (tested every ``check_every`` iterations).
+
+        With ``graph=True`` a solve still running after ``graph_after``
+        blocks of ``check_every`` eager iterations continues as replays of
+        one HIP graph holding ``check_every`` iterations (the ~15 launches of
+        an iteration cost more than their work on meshes of a few thousand
+        nodes; capturing costs about one block, so short solves stay eager).
+        The graph runs the same kernels in the same order on the same
+        buffers, so the iterates are bitwise those of ``graph=False``."""
         t = _lib.torch()
         m = self.mask
         B = B * m
@@ -100,24 +108,45 @@ class Linearization:
         else:
             def apply(V):
                 return V.clone()
-        Z = apply(R)
-        P = Z.clone()
-        rz = (R.double() * Z.double()).sum(dim=(1, 2))
+        P = apply(R)
+        rz = (R.double() * P.double()).sum(dim=(1, 2))
         stop = (rtol ** 2) * (R.double() ** 2).sum(dim=(1, 2)).clamp_min(1e-300)
-        for it in range(max_iter):
+
+        def step(update_p: bool):
             HP = self.hvp(P, alpha) * m
             php = (P.double() * HP.double()).sum(dim=(1, 2))
             a = t.where(php > 0, rz / php.clamp_min(1e-300), t.zeros_like(rz)).float()[:, None, None]
             X.addcmul_(a, P)
             R.addcmul_(a, HP, value=-1.0)
-            if (it + 1) % check_every == 0 and bool(((R.double() ** 2).sum(dim=(1, 2)) <= stop).all()):
+            if update_p:
+                Z = apply(R)
+                rz_new = (R.double() * Z.double()).sum(dim=(1, 2))
+                beta = t.where(rz > 0, rz_new / rz.clamp_min(1e-300), t.zeros_like(rz)).float()[:, None, None]
+                P.copy_(t.addcmul(Z, beta, P))
+                rz.copy_(rz_new)
+
+        def converged():
+            return bool(((R.double() ** 2).sum(dim=(1, 2)) <= stop).all())
+
+        k = max(int(check_every), 1)
+        it = 0
+        g = None
+        while it < max_iter:
+            n = min(k, max_iter - it)
+            if graph and g is None and it >= max(int(graph_after), 1) * k and n == k:
+                g = t.cuda.CUDAGraph()
+                with t.cuda.graph(g):
+                    for _ in range(k):
+                        step(True)
+            if g is not None and n == k:
+                g.replay()
+            else:
+                for _ in range(n):
+                    step(True)
+            it += n
+            if n == k and converged():
                 break
-            Z = apply(R)
-            rz_new = (R.double() * Z.double()).sum(dim=(1, 2))
-            beta = t.where(rz > 0, rz_new / rz.clamp_min(1e-300), t.zeros_like(rz)).float()[:, None, None]
-            P = t.addcmul(Z, beta, P)
-            rz = rz_new
-        self.last_iters = it + 1
+        self.last_iters = it
         return X
 
     def tangent_rhs(self, dimg, which: str = "g"):
@@ -211,6 +240,52 @@ def solve_differentiable(plan: CGPlan, ref_image, def_image, reg_strength, u0=No
     return _FN.apply(ref_image, def_image, reg_strength, plan, u0, max_iter, tol, interpolation, opts)
 
 
+_ROW_GATHER = None
+
+
+def _row_gather_fn():
+    """u[..., idx, :] whose backward is a fixed-order gather over the
+    transpose of idx (torch's own backward of that indexing accumulates the
+    repeated node indices serially and took ~6 ms for 4,225 nodes)."""
+    t = _lib.torch()
+
+    class RowGather(t.autograd.Function):
+        @staticmethod
+        def forward(ctx, u, flat_idx, tpos):
+            # u has one row more than tpos (the zero row that padding slots read)
+            ctx.save_for_backward(tpos)
+            return u.index_select(-2, flat_idx)
+
+        @staticmethod
+        def backward(ctx, g):
+            (tpos,) = ctx.saved_tensors
+            n, width = int(tpos.shape[0]), int(tpos.shape[1])
+            gz = t.cat([g, t.zeros_like(g[..., :1, :])], dim=-2)  # position m reads zero
+            out = gz.index_select(-2, tpos.reshape(-1)).reshape(*g.shape[:-2], n, width, g.shape[-1]).sum(-2)
+            return t.cat([out, t.zeros_like(out[..., :1, :])], dim=-2), None, None
+
+    return RowGather
+
+
+def _gather_tables(idx, valid, n_rows):
+    """flat row index per (node, slot) (padding -> the appended zero row
+    n_rows) and the transpose table: for each row, the flat positions that
+    read it in ascending order, padded with the zero-row position."""
+    t = _lib.torch()
+    flat = t.where(valid, idx, t.full_like(idx, n_rows)).reshape(-1)
+    m = flat.numel()
+    keys, pos = t.sort(flat, stable=True)
+    cnt = t.bincount(flat, minlength=n_rows + 1)[:n_rows]
+    start = t.cumsum(cnt, 0) - cnt
+    real = keys < n_rows
+    keys, pos = keys[real], pos[real]
+    rank = t.arange(keys.numel(), device=idx.device) - start[keys]
+    width = max(int(cnt.max().item()) if n_rows > 0 else 0, 1)
+    tpos = t.full((n_rows, width), m, dtype=t.int64, device=idx.device)
+    tpos[keys, rank] = pos
+    return flat, tpos
+
+
 def green_lagrange_torch(u, nodes, nb_index, nb_degree, gauge_length: float = 0.0, eps: float = 1e-8):
     """Nodal LSQ Green-Lagrange tensor (``strain.py:11-114``) as differentiable
     torch code: returns E (..., n_nodes, 2, 2) for u (..., n_nodes, 2)."""
@@ -221,7 +296,14 @@ def green_lagrange_torch(u, nodes, nb_index, nb_degree, gauge_length: float = 0.
     md = idx.shape[1]
     valid = (t.arange(md, device=idx.device)[None, :] < deg[:, None])
     dX = X[idx] - X[:, None, :]
-    dU = u[..., idx, :] - u[..., :, None, :]
+    global _ROW_GATHER
+    if _ROW_GATHER is None:
+        _ROW_GATHER = _row_gather_fn()
+    nn = int(u.shape[-2])
+    flat, tpos = _gather_tables(idx, valid, nn)
+    u_ext = t.cat([u, t.zeros_like(u[..., :1, :])], dim=-2)
+    uj = _ROW_GATHER.apply(u_ext, flat, tpos).reshape(*u.shape[:-2], nn, md, u.shape[-1])
+    dU = uj - u[..., :, None, :]
     if gauge_length > 0:
         w = t.exp(-((dX ** 2).sum(-1)) / (gauge_length ** 2))
     else:
@@ -229,7 +311,12 @@ def green_lagrange_torch(u, nodes, nb_index, nb_degree, gauge_length: float = 0.
     w = w * valid.to(u.dtype)
     A = t.einsum("nk,nki,nkj->nij", w, dX, dX) + eps * t.eye(2, dtype=u.dtype, device=u.device)
     Bm = t.einsum("nk,...nki,nkj->...nij", w, dU, dX)  # rows: component of u, cols: direction
-    grad_u = t.linalg.solve(A, Bm.transpose(-1, -2)).transpose(-1, -2)
+    # grad_u[r] = A^-1 Bm[r] for each component r (the reference's two 2x2
+    # solves), by Cramer's rule: elementwise, so its backward is elementwise too
+    a, b, c, d = A[:, 0, 0], A[:, 0, 1], A[:, 1, 0], A[:, 1, 1]
+    det = a * d - b * c
+    B0, B1 = Bm[..., 0], Bm[..., 1]
+    grad_u = t.stack([(d[:, None] * B0 - b[:, None] * B1), (a[:, None] * B1 - c[:, None] * B0)], -1) / det[:, None, None]
     F = t.eye(2, dtype=u.dtype, device=u.device) + grad_u
     return 0.5 * (F.transpose(-1, -2) @ F - t.eye(2, dtype=u.dtype, device=u.device))
 

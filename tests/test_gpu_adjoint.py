@@ -234,3 +234,30 @@ def test_block_jacobi_preconditioner(gpu):
     rel = float((x1 - x0).norm() / x0.norm())
     print(f"checked {checked} blocks; iterations plain {it0}, preconditioned {it1}; solution rel diff {rel:.2e}")
     assert rel < 1e-4
+
+
+def test_graph_captured_cg_equals_eager(gpu):
+    """The linear CG's HIP-graph replays (check_every iterations per replay)
+    give bitwise the eager iterates, over several replays per solve and
+    several solves with eager work and synchronisations in between."""
+    from d2ic_amd.adjoint import Linearization
+
+    print("[case] adjoint CG: graph replays vs eager", flush=True)
+    m, mesh, ref, de, plan, oa, _, _ = _setup()
+    t = gpu
+    u = _u64(ref, de, oa, np.zeros((plan.n_nodes, 2)), newton=1)
+    lin = Linearization(plan, de, u.astype(np.float32))
+    act = plan.node_active.cpu().numpy()
+    rng = np.random.default_rng(5)
+    b = t.tensor(rng.normal(size=(3, plan.n_nodes, 2)) * act[None, :, None], dtype=t.float32, device="cuda")
+    for pre, every in ((False, 10), (True, 5)):
+        xe = lin.solve(b, ALPHA, rtol=1e-8, precondition=pre, check_every=every, graph=False)
+        ite = lin.last_iters
+        assert ite >= 4 * every  # at least three replays after the eager block
+        for rep in range(3):
+            xg = lin.solve(b, ALPHA, rtol=1e-8, precondition=pre, check_every=every, graph=True, graph_after=1)
+            print(f"precondition={pre} check_every={every} rep {rep}: iterations {lin.last_iters} (eager {ite})")
+            assert lin.last_iters == ite
+            assert t.equal(xg, xe)
+            _ = lin.hvp(b, ALPHA)  # eager work between the graph solves
+            t.cuda.synchronize()
