@@ -146,7 +146,7 @@ struct PlanEntry {
 };
 
 constexpr int kMaxBlocks = 1024;
-constexpr int kArriveStride = 32;  // one 128-byte line per block's arrival flag
+constexpr int kArriveStride = 32;  // one 128-byte line per block's release word
 // release word: step + 1 (mod 2^30) in the low bits, abort and session-stop flags above
 constexpr unsigned kReleaseAbort = 0x80000000u;
 constexpr unsigned kReleaseStop = 0x40000000u;
@@ -179,7 +179,10 @@ struct Workspace {
     int bad[3][kMaxProblems];
     PlanEntry plan[kMaxProblems];  // first step's plan (written by k_init)
     int n_tiles_active, n_chunks_active, any_active, pad0;
-    unsigned arrive[kMaxBlocks * kArriveStride];   // last step each block finished
+    // last step each block finished: block b's word is arrive[b], packed so
+    // that the master's poll reads 8 lines, not one line per block (1 %
+    // per iteration); the rest of the array only keeps release[] where it was
+    unsigned arrive[kMaxBlocks * kArriveStride];
     unsigned release[kMaxBlocks * kArriveStride];  // last step every block finished (one line per block)
     unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
     int stop;                                      // served mode: every block leaves after this barrier
@@ -1282,7 +1285,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             // used before the block arrived.)
             const bool wrote = blockIdx.x == 0 || (isA && nT > 0) || ci0 < ci1;
             if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
+            __hip_atomic_store(&ws->arrive[blockIdx.x], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             s_ok = 1;
             s_stop = 0;
@@ -1304,7 +1307,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const int b = b0 + j * kWave + lane;
-                            v[j] = b < nb ? __hip_atomic_load(&ws->arrive[b * kArriveStride], __ATOMIC_RELAXED,
+                            v[j] = b < nb ? __hip_atomic_load(&ws->arrive[b], __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_AGENT)
                                           : (unsigned)(step + 1);
                         }
