@@ -378,10 +378,12 @@ def differentiation_leg(seed=3, n_probes=256, max_iters=200, tol=1e-3, reg=0.1):
 def local_refine_leg(dic, ref, frame, u0, n_pixels, sweeps=3, reps=5):
     """LocalGaussNewtonSolver on the headline workload (the reference
     tutorial's refinement: 3 sweeps, lam 0.1, max_step 0.2, omega 0.5) from the
-    global solution.  Algorithmic bytes per sweep: pixel pass 56 B/px (coords,
-    4 node ids, 4 shape values, f, and r, dg/dx, dg/dy written) + node pass
-    20 B per node->pixel entry (index, weight, gathered r, dg/dx, dg/dy) with
-    4 entries per pixel."""
+    global solution.  Algorithmic bytes per sweep are those of the reference's
+    layout: pixel pass 56 B/px (coords, 4 node ids, 4 shape values, f, and r,
+    dg/dx, dg/dy written) + node pass 20 B per node->pixel entry (index,
+    weight, gathered r, dg/dx, dg/dy) with 4 entries per pixel.  The solver
+    runs on the element tiles (per-tile node sums), which reads ~36 B/px, so
+    this figure can exceed what the kernels move."""
     import torch
 
     from d2ic_amd import DICMeshBased, LocalGaussNewtonSolver, MeshDICConfig

@@ -58,6 +58,8 @@ _SIGS = {
     "d2ic_adj_diag_inv": [_vp, _vp, _f32, _vp, _vp],
     "d2ic_lgn_solve": [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _i32,
                        _i32, _i32, _vp, _i32, _f32, _f32, _f32, _f32, _vp, _vp],
+    "d2ic_lgn_solve_plan": [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i32, _i32, _i32, _vp, _i32, _f32, _f32, _f32, _f32,
+                            _vp, _vp],
     "d2ic_zncc_match": [_vp, _i64, _vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp],
     "d2ic_idw": [_vp, _i64, _vp, _vp, _i64, _vp, _vp],
     "d2ic_strain": [_vp, _vp, _i64, _vp, _vp, _i32, _f32, _f32, _vp, _vp, _vp, _vp],

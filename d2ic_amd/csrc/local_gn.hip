@@ -12,14 +12,22 @@
 //     1e-8 I), clipped to |delta| <= max_step, u += omega delta.
 // History per sweep: RMS residual over the pixels and RMS displacement update.
 //
-// Layout: the reference's own pixel order and node->pixel table (the solver
-// reads them as built by build_pixel_assets); one thread per pixel, one wave
-// per node.  Sampling positions use the split integer-cell + fraction form of
-// the global solver (more accurate than x + u in FP32 at large x).
+// Two layouts:
+//   * d2ic_lgn_solve: the reference's own pixel order and node->pixel table
+//     (as built by build_pixel_assets); one thread per pixel, one wave per
+//     node walking the node's table row (20 B per entry, 4 entries per pixel);
+//   * d2ic_lgn_solve_plan: the global solver's plan (pixels grouped by element
+//     into tiles of <= 256); one block per tile reduces the tile's pixels to
+//     5 sums (g0, g1, h00, h01, h11) per tile node, and a node gathers its
+//     <= 4 (or 16 for split elements) tile sums through the plan's node->tile
+//     table, so the node pass reads ~20 B per tile entry instead of per pixel.
+// Sampling positions use the split integer-cell + fraction form of the global
+// solver (more accurate than x + u in FP32 at large x).
 #include <algorithm>
 
 #include "common.hpp"
 #include "interp.hpp"
+#include "plan.hpp"
 
 namespace d2ic {
 
@@ -98,6 +106,46 @@ __global__ __launch_bounds__(kLgnBlock) void k_lgn_pixels(const float2* __restri
     block_sum_store(r2, r2_part + blockIdx.x);
 }
 
+// The 2x2 update of node n from its image sums (g, H) and the spring term
+// (jacobi_nodal_step_spring, solver_local_gn.py:312-414); returns |step|^2.
+__device__ __forceinline__ double lgn_node_update(int64_t n, const float2* __restrict__ u, float g0, float g1, float h00,
+                                                  float h01, float h11, bool has_pixels,
+                                                  const int32_t* __restrict__ nbi, const int32_t* __restrict__ nbd,
+                                                  const float* __restrict__ nbw, const float* __restrict__ nrw,
+                                                  int64_t mdn, float lam, float max_step, float alpha, float omega,
+                                                  float2* __restrict__ u_next) {
+    const float2 ui = u[n];
+    const int dn = nbd[n];
+    float gr0 = 0.f, gr1 = 0.f, sumw = 0.f;
+    for (int k = 0; k < dn; ++k) {
+        const float w = nbw[n * mdn + k];
+        const float2 uj = u[nbi[n * mdn + k]];
+        gr0 += w * (ui.x - uj.x);
+        gr1 += w * (ui.y - uj.y);
+        sumw += w;
+    }
+    const float alpha_loc = alpha * nrw[n];
+    const float lam_loc = lam * (h00 + h11 + 1e-12f);
+    const float hreg = alpha_loc * sumw;
+    const float a = h00 + lam_loc + hreg + 1e-8f, b = h01, d = h11 + lam_loc + hreg + 1e-8f;
+    const float q0 = g0 + alpha_loc * gr0, q1 = g1 + alpha_loc * gr1;
+    const bool has_update = has_pixels || (alpha != 0.f && dn > 0);
+    float d0 = 0.f, d1 = 0.f;
+    if (has_update) {
+        const float det = a * d - b * b;
+        d0 = -(d * q0 - b * q1) / det;
+        d1 = -(a * q1 - b * q0) / det;
+        const float nrm = sqrtf(d0 * d0 + d1 * d1);
+        const float factor = fminf(1.0f, max_step / (nrm + 1e-12f));
+        d0 *= factor;
+        d1 *= factor;
+    }
+    const float2 un = make_float2(ui.x + omega * d0, ui.y + omega * d1);
+    u_next[n] = un;
+    const float sx = un.x - ui.x, sy = un.y - ui.y;
+    return (double)sx * (double)sx + (double)sy * (double)sy;
+}
+
 // One wave per node: lanes stride over the node's pixels (FP32 sums, DPP
 // reduction); the last lane adds the spring term and solves the 2x2 system.
 __global__ __launch_bounds__(kLgnBlock) void k_lgn_nodes(
@@ -127,37 +175,114 @@ __global__ __launch_bounds__(kLgnBlock) void k_lgn_nodes(
     h01 = wave_sum63(h01);
     h11 = wave_sum63(h11);
     double s2 = 0.0;
-    if (lane == kWave - 1 && n < nn) {
-        const float2 ui = u[n];
-        const int dn = nbd[n];
-        float gr0 = 0.f, gr1 = 0.f, sumw = 0.f;
-        for (int k = 0; k < dn; ++k) {
-            const float w = nbw[n * mdn + k];
-            const float2 uj = u[nbi[n * mdn + k]];
-            gr0 += w * (ui.x - uj.x);
-            gr1 += w * (ui.y - uj.y);
-            sumw += w;
+    if (lane == kWave - 1 && n < nn)
+        s2 = lgn_node_update(n, u, g0, g1, h00, h01, h11, deg > 0, nbi, nbd, nbw, nrw, mdn, lam, max_step, alpha, omega,
+                             u_next);
+    block_sum_store(s2, s2_part + blockIdx.x);
+}
+
+// ---- plan (tile) layout ----
+static_assert(kTile == kLgnBlock, "one thread per tile slot");
+
+// f at every tile slot (padded slots are never read)
+__global__ void k_lgn_ref_tiles(PlanView pv, const float* __restrict__ img, int H, int W, int interp,
+                                float* __restrict__ I1) {
+    const int64_t t = blockIdx.x;
+    const int i = threadIdx.x;
+    if (i >= pv.tile_cnt[t]) return;
+    const int64_t s = t * kTile + i;
+    int x0, y0;
+    float fx, fy;
+    split_coord(pv.px[s], 0.f, x0, fx);
+    split_coord(pv.py[s], 0.f, y0, fy);
+    I1[s] = sample_value(interp, img, H, W, x0, fx, y0, fy);
+}
+
+// One block per tile: the pixel state (as k_lgn_pixels) and the tile's sums
+// (g0, g1, h00, h01, h11) for each of its 4 nodes, reduced in a fixed order
+// (wave DPP sums, then the 4 waves in order); r^2 per tile in FP64.
+__global__ __launch_bounds__(kLgnBlock) void k_lgn_tiles(PlanView pv, const float2* __restrict__ u,
+                                                         const float* __restrict__ img, const float* __restrict__ gximg,
+                                                         const float* __restrict__ gyimg, int H, int W, int interp,
+                                                         const float* __restrict__ I1, float* __restrict__ tsum,
+                                                         double* __restrict__ r2_part) {
+    __shared__ float s_v[kLgnBlock / kWave][20];
+    __shared__ double s_r2[kLgnBlock / kWave];
+    const int64_t t = blockIdx.x;
+    const int i = threadIdx.x, lane = i & (kWave - 1), wave = i / kWave;
+    const int4 q = pv.tile_nodes[t];
+    float v[20];
+#pragma unroll
+    for (int k = 0; k < 20; ++k) v[k] = 0.f;
+    double r2 = 0.0;
+    if (i < pv.tile_cnt[t]) {
+        const int64_t s = t * kTile + i;
+        const float4 w = pv.N4[s];
+        const float2 ua = u[q.x], ub = u[q.y], uc = u[q.z], ud = u[q.w];
+        const float ux = w.x * ua.x + w.y * ub.x + w.z * uc.x + w.w * ud.x;
+        const float uy = w.x * ua.y + w.y * ub.y + w.z * uc.y + w.w * ud.y;
+        int x0, y0;
+        float fx, fy;
+        split_coord(pv.px[s], ux, x0, fx);
+        split_coord(pv.py[s], uy, y0, fy);
+        const float rv = sample_value(interp, img, H, W, x0, fx, y0, fy) - I1[s];
+        const float gx = sample_value(interp, gximg, H, W, x0, fx, y0, fy);
+        const float gy = sample_value(interp, gyimg, H, W, x0, fx, y0, fy);
+        const float Na[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float jx = gx * Na[a], jy = gy * Na[a];
+            v[5 * a + 0] = rv * jx;
+            v[5 * a + 1] = rv * jy;
+            v[5 * a + 2] = jx * jx;
+            v[5 * a + 3] = jx * jy;
+            v[5 * a + 4] = jy * jy;
         }
-        const float alpha_loc = alpha * nrw[n];
-        const float lam_loc = lam * (h00 + h11 + 1e-12f);
-        const float hreg = alpha_loc * sumw;
-        const float a = h00 + lam_loc + hreg + 1e-8f, b = h01, d = h11 + lam_loc + hreg + 1e-8f;
-        const float q0 = g0 + alpha_loc * gr0, q1 = g1 + alpha_loc * gr1;
-        const bool has_update = deg > 0 || (alpha != 0.f && dn > 0);
-        float d0 = 0.f, d1 = 0.f;
-        if (has_update) {
-            const float det = a * d - b * b;
-            d0 = -(d * q0 - b * q1) / det;
-            d1 = -(a * q1 - b * q0) / det;
-            const float nrm = sqrtf(d0 * d0 + d1 * d1);
-            const float factor = fminf(1.0f, max_step / (nrm + 1e-12f));
-            d0 *= factor;
-            d1 *= factor;
+        r2 = (double)rv * (double)rv;
+    }
+#pragma unroll
+    for (int k = 0; k < 20; ++k) v[k] = wave_sum63(v[k]);
+    r2 = wave_sum63(r2);
+    if (lane == kWave - 1) {
+#pragma unroll
+        for (int k = 0; k < 20; ++k) s_v[wave][k] = v[k];
+        s_r2[wave] = r2;
+    }
+    __syncthreads();
+    if (i < 20) {
+        float acc = 0.f;
+        for (int w2 = 0; w2 < kLgnBlock / kWave; ++w2) acc += s_v[w2][i];
+        tsum[t * 20 + i] = acc;
+    } else if (i == 20) {
+        double acc = 0.0;
+        for (int w2 = 0; w2 < kLgnBlock / kWave; ++w2) acc += s_r2[w2];
+        r2_part[t] = acc;
+    }
+}
+
+// One thread per node: the node's tile sums gathered in the table's order,
+// then the 2x2 update.
+__global__ __launch_bounds__(kLgnBlock) void k_lgn_nodes_tiles(
+    PlanView pv, const float2* __restrict__ u, const float* __restrict__ tsum, const int32_t* __restrict__ nbi,
+    const int32_t* __restrict__ nbd, const float* __restrict__ nbw, const float* __restrict__ nrw, int64_t mdn,
+    int64_t nn, float lam, float max_step, float alpha, float omega, float2* __restrict__ u_next,
+    double* __restrict__ s2_part) {
+    const int64_t n = (int64_t)blockIdx.x * kLgnBlock + threadIdx.x;
+    double s2 = 0.0;
+    if (n < nn) {
+        const int e0 = pv.node_ptr[n], e1 = pv.node_ptr[n + 1];
+        float g0 = 0.f, g1 = 0.f, h00 = 0.f, h01 = 0.f, h11 = 0.f;
+        for (int e = e0; e < e1; ++e) {
+            const int ent = pv.node_ent[e];
+            const float* b = tsum + (int64_t)(ent >> 2) * 20 + 5 * (ent & 3);
+            g0 += b[0];
+            g1 += b[1];
+            h00 += b[2];
+            h01 += b[3];
+            h11 += b[4];
         }
-        const float2 un = make_float2(ui.x + omega * d0, ui.y + omega * d1);
-        u_next[n] = un;
-        const float sx = un.x - ui.x, sy = un.y - ui.y;
-        s2 = (double)sx * (double)sx + (double)sy * (double)sy;
+        s2 = lgn_node_update(n, u, g0, g1, h00, h01, h11, e1 > e0, nbi, nbd, nbw, nrw, mdn, lam, max_step, alpha,
+                             omega, u_next);
     }
     block_sum_store(s2, s2_part + blockIdx.x);
 }
@@ -245,6 +370,67 @@ int d2ic_lgn_solve(const float* pixel_coords, const int32_t* pixel_nodes, const 
         D2IC_LAUNCH_CHECK();
         if (hist) {
             k_lgn_hist<<<1, kLgnBlock, 0, st>>>(r2_part, bp, s2_part, bn, np, nn, hist + 2 * (size_t)k);
+            D2IC_LAUNCH_CHECK();
+        }
+        std::swap(ucur, unext);
+    }
+    if (ucur != reinterpret_cast<float2*>(u))
+        D2IC_HIP(hipMemcpyAsync(u, ucur, sizeof(float) * 2 * nn, hipMemcpyDeviceToDevice, st));
+    D2IC_HIP(hipFreeAsync(mem, st));
+    return D2IC_OK;
+}
+
+int d2ic_lgn_solve_plan(const d2ic_cg_plan* plan, const int32_t* nb_idx, const int32_t* nb_deg, const float* nb_w,
+                        const float* node_reg_weight, int64_t max_deg_nb, const float* ref_image,
+                        const float* def_image, int32_t H, int32_t W, int32_t interp, float* u, int32_t n_sweeps,
+                        float lam, float max_step, float alpha_reg, float omega, float* hist, void* stream) {
+    D2IC_CHECK_ARG(plan && u && ref_image && def_image, "plan, u and the images must not be NULL");
+    D2IC_CHECK_ARG(H > 0 && W > 0 && n_sweeps >= 0, "sizes must be non-negative");
+    D2IC_CHECK_ARG(interp == 0 || interp == 1, "interp must be 0 (cubic) or 1 (linear)");
+    const int64_t nn = plan->nn, nt = plan->ntiles;
+    D2IC_CHECK_ARG(nn == 0 || (nb_deg && node_reg_weight), "node arrays must not be NULL");
+    D2IC_CHECK_ARG(max_deg_nb == 0 || (nb_idx && nb_w), "neighbour table must not be NULL");
+    hipStream_t st = (hipStream_t)stream;
+    if (n_sweeps == 0 || nn == 0) return D2IC_OK;
+    const PlanView pv = plan_view(plan);
+    const int64_t hw = (int64_t)H * W, bn = ceil_div(nn, kLgnBlock), nslot = std::max<int64_t>(nt * kTile, 1);
+    const size_t bytes = sizeof(float) * (2 * hw + nslot + 20 * std::max<int64_t>(nt, 1) + 2 * nn) +
+                         sizeof(double) * (std::max<int64_t>(nt, 1) + bn) + 256;
+    void* mem = nullptr;
+    D2IC_HIP(hipMallocAsync(&mem, bytes, st));
+    char* cur = reinterpret_cast<char*>(mem);
+    auto take = [&](size_t b) {
+        char* p = cur;
+        cur += (b + 15) / 16 * 16;
+        return p;
+    };
+    double* r2_part = reinterpret_cast<double*>(take(sizeof(double) * std::max<int64_t>(nt, 1)));
+    double* s2_part = reinterpret_cast<double*>(take(sizeof(double) * bn));
+    float* gximg = reinterpret_cast<float*>(take(sizeof(float) * hw));
+    float* gyimg = reinterpret_cast<float*>(take(sizeof(float) * hw));
+    float* I1 = reinterpret_cast<float*>(take(sizeof(float) * nslot));
+    float* tsum = reinterpret_cast<float*>(take(sizeof(float) * 20 * std::max<int64_t>(nt, 1)));
+    float2* ub = reinterpret_cast<float2*>(take(sizeof(float) * 2 * nn));
+    k_lgn_grad<<<(unsigned)ceil_div(hw, 256), 256, 0, st>>>(def_image, H, W, gximg, gyimg);
+    D2IC_LAUNCH_CHECK();
+    if (nt > 0) {
+        k_lgn_ref_tiles<<<(unsigned)nt, kLgnBlock, 0, st>>>(pv, ref_image, H, W, interp, I1);
+        D2IC_LAUNCH_CHECK();
+    }
+    float2* ucur = reinterpret_cast<float2*>(u);
+    float2* unext = ub;
+    for (int k = 0; k < n_sweeps; ++k) {
+        if (nt > 0) {
+            k_lgn_tiles<<<(unsigned)nt, kLgnBlock, 0, st>>>(pv, ucur, def_image, gximg, gyimg, H, W, interp, I1, tsum,
+                                                            r2_part);
+            D2IC_LAUNCH_CHECK();
+        }
+        k_lgn_nodes_tiles<<<(unsigned)bn, kLgnBlock, 0, st>>>(pv, ucur, tsum, nb_idx, nb_deg, nb_w, node_reg_weight,
+                                                               max_deg_nb, nn, lam, max_step, alpha_reg, omega, unext,
+                                                               s2_part);
+        D2IC_LAUNCH_CHECK();
+        if (hist) {
+            k_lgn_hist<<<1, kLgnBlock, 0, st>>>(r2_part, nt, s2_part, bn, plan->np, nn, hist + 2 * (size_t)k);
             D2IC_LAUNCH_CHECK();
         }
         std::swap(ucur, unext);

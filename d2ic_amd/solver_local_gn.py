@@ -11,6 +11,11 @@ gradient images of the deformed image (``_local_sweeps`` ``:205-309``,
 ``interpolation``: "cubic" (Keys a = -1/2, as in ``GlobalCGSolver``; the
 reference delegates cubic to im_jax, which is not available) or
 "linear"/"bilinear" (the reference's ``_bilinear_sample``, ``:417-442``).
+
+``layout``: "tiles" (default) runs the sweeps on the global solver's element
+tiles (``d2ic_lgn_solve_plan``: per-tile node sums, then a node gathers its
+tile sums); "table" walks the reference's node->pixel table
+(``d2ic_lgn_solve``).  Both give the same result up to FP32 summation order.
 """
 from __future__ import annotations
 
@@ -20,7 +25,7 @@ from typing import Any
 
 from . import _lib
 from .solver_base import SolverBase
-from .solver_global_cg import _normalize_interpolation
+from .solver_global_cg import CGPlan, _normalize_interpolation
 from .types import Array
 
 
@@ -37,7 +42,11 @@ class LocalGaussNewtonSolver(SolverBase):
     """Local Gauss-Newton solver for mesh DIC (nodal local updates)."""
 
     def __init__(self, lam: float = 1e-3, max_step: float = 0.2, omega: float = 0.5, interpolation: str = "cubic",
-                 use_map_coordinates: bool | None = None) -> None:
+                 use_map_coordinates: bool | None = None, layout: str = "tiles") -> None:
+        if layout not in ("tiles", "table"):
+            raise ValueError("layout must be 'tiles' or 'table'")
+        self._layout = layout
+        self._plan = None
         self._compiled = False
         self._lam = float(lam)
         self._max_step = float(max_step)
@@ -76,6 +85,7 @@ class LocalGaussNewtonSolver(SolverBase):
             if d[k].dim() != 2 or int(d[k].shape[0]) != nn:
                 d[k] = d[k].reshape(nn, -1).contiguous()
         self._dev = d
+        self._plan = CGPlan(pix) if self._layout == "tiles" else None
         self._compiled = True
 
     def solve(self, state: Any, def_image: Array) -> LocalGNResult:
@@ -99,12 +109,19 @@ class LocalGaussNewtonSolver(SolverBase):
         cfg = state.config
         n_sweeps = int(cfg.max_iters)
         hist = t.full((max(n_sweeps, 0), 2), float("nan"), dtype=t.float32, device=_lib.device())
-        _lib.call("d2ic_lgn_solve", _lib.ptr(d["coords"]), _lib.ptr(d["pnodes"]), _lib.ptr(d["shapeN"]),
-                  int(d["coords"].shape[0]), _lib.ptr(d["npi"]), _lib.ptr(d["npw"]), _lib.ptr(d["npd"]),
-                  int(d["npi"].shape[1]), _lib.ptr(d["nbi"]), _lib.ptr(d["nbd"]), _lib.ptr(d["nbw"]),
-                  _lib.ptr(d["nrw"]), int(d["nbi"].shape[1]), nn, _lib.ptr(ref), _lib.ptr(de), H, W,
-                  _lib.INTERP[self._interpolation], _lib.ptr(u), n_sweeps, self._lam, self._max_step,
-                  float(cfg.reg_strength), self._omega, _lib.ptr(hist) if n_sweeps > 0 else None, _lib.stream())
+        hist_p = _lib.ptr(hist) if n_sweeps > 0 else None
+        if self._plan is not None:
+            _lib.call("d2ic_lgn_solve_plan", self._plan.handle, _lib.ptr(d["nbi"]), _lib.ptr(d["nbd"]),
+                      _lib.ptr(d["nbw"]), _lib.ptr(d["nrw"]), int(d["nbi"].shape[1]), _lib.ptr(ref), _lib.ptr(de), H,
+                      W, _lib.INTERP[self._interpolation], _lib.ptr(u), n_sweeps, self._lam, self._max_step,
+                      float(cfg.reg_strength), self._omega, hist_p, _lib.stream())
+        else:
+            _lib.call("d2ic_lgn_solve", _lib.ptr(d["coords"]), _lib.ptr(d["pnodes"]), _lib.ptr(d["shapeN"]),
+                      int(d["coords"].shape[0]), _lib.ptr(d["npi"]), _lib.ptr(d["npw"]), _lib.ptr(d["npd"]),
+                      int(d["npi"].shape[1]), _lib.ptr(d["nbi"]), _lib.ptr(d["nbd"]), _lib.ptr(d["nbw"]),
+                      _lib.ptr(d["nrw"]), int(d["nbi"].shape[1]), nn, _lib.ptr(ref), _lib.ptr(de), H, W,
+                      _lib.INTERP[self._interpolation], _lib.ptr(u), n_sweeps, self._lam, self._max_step,
+                      float(cfg.reg_strength), self._omega, hist_p, _lib.stream())
         strain = t.zeros((nn, 3), dtype=u.dtype, device=u.device)
         save_history = bool(getattr(cfg, "save_history", False))
         return LocalGNResult(u_nodal=u, strain=strain, history=hist if save_history else None)

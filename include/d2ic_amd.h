@@ -210,6 +210,18 @@ int d2ic_lgn_solve(const float* pixel_coords, const int32_t* pixel_nodes, const 
                    const float* def_image, int32_t H, int32_t W, int32_t interp, float* u, int32_t n_sweeps, float lam,
                    float max_step, float alpha_reg, float omega, float* hist, void* stream);
 
+/* The same sweeps on a solver plan's layout (d2ic_cg_plan_create from the same
+ * PixelAssets): per sweep one block per element tile reduces the tile's pixels
+ * to 5 sums per tile node, and each node gathers its tile sums through the
+ * plan's node->tile table instead of walking its node->pixel row.  Same
+ * arguments as d2ic_lgn_solve minus the pixel arrays and the node->pixel
+ * table (the plan holds them); results agree with d2ic_lgn_solve to FP32
+ * summation order. */
+int d2ic_lgn_solve_plan(const d2ic_cg_plan* plan, const int32_t* nb_idx, const int32_t* nb_deg, const float* nb_w,
+                        const float* node_reg_weight, int64_t max_deg_nb, const float* ref_image,
+                        const float* def_image, int32_t H, int32_t W, int32_t interp, float* u, int32_t n_sweeps,
+                        float lam, float max_step, float alpha_reg, float omega, float* hist, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Translation ZNCC (replaces _match_centers_jax, solver_translation_zncc.py:118-168)
  * ------------------------------------------------------------------------ */

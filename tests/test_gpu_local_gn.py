@@ -78,3 +78,37 @@ def test_local_gn_zero_sweeps_and_errors(gpu):
     sol = dic.solver.solve(dic.state, de)
     np.testing.assert_array_equal(sol.u_nodal.cpu().numpy(), u0)
     assert tuple(sol.history.shape) == (0, 2)
+
+
+@pytest.mark.parametrize("interp", ["cubic", "linear"])
+def test_tile_layout_matches_table_layout(gpu, interp):
+    """d2ic_lgn_solve_plan (per-tile node sums over the solver plan's element
+    tiles) against d2ic_lgn_solve (the reference's node->pixel table): same
+    sweeps up to FP32 summation order, on a mesh with 32 px elements (tiles
+    split into 16-column strips) and one with 8 px elements."""
+    from d2ic_amd import DICMeshBased, LocalGaussNewtonSolver, MeshDICConfig, mask_to_mesh_assets, speckle
+
+    t = gpu
+    for e in (32, 8):
+        print(f"[case] local GN tiles vs table, e={e}, {interp}", flush=True)
+        m = speckle.rect_mask(H, W, 64, 64)
+        mesh, assets = mask_to_mesh_assets(m, e)
+        fn = speckle.affine_field([[2e-3, 1e-3], [-1e-3, 3e-3]], [0.35, -0.25], 0.2, 80.0, W / 2, H / 2)
+        ref, de = speckle.speckle_pair(H, W, 11, fn, noise_sigma=0.5)
+        nn = mesh.nodes_xy.shape[0]
+        u0 = t.tensor(np.tile([0.3, -0.2], (nn, 1)).astype(np.float32), device="cuda")
+        out = {}
+        for layout in ("tiles", "table"):
+            dic = DICMeshBased(mesh, LocalGaussNewtonSolver(lam=0.1, max_step=0.2, omega=0.5, interpolation=interp,
+                                                            layout=layout),
+                               MeshDICConfig(max_iters=4, tol=1e-3, reg_strength=0.1, save_history=True))
+            dic.prepare(ref, assets)
+            dic.set_initial_guess(u0.clone())
+            sol = dic.solver.solve(dic.state, de)
+            out[layout] = (sol.u_nodal.cpu().numpy(), sol.history.cpu().numpy())
+        du = np.abs(out["tiles"][0] - out["table"][0]).max()
+        moved = np.abs(out["table"][0] - u0.cpu().numpy()).max()
+        print(f"e={e}: max |u_tiles - u_table| = {du:.2e} px (update {moved:.3f} px)")
+        assert moved > 1e-2
+        assert du < 2e-6
+        np.testing.assert_allclose(out["tiles"][1], out["table"][1], rtol=1e-5, atol=1e-7)
