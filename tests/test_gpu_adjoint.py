@@ -20,7 +20,7 @@ H = W = 96
 ALPHA = 0.1
 
 
-def _setup():
+def _setup(e=16):
     from d2ic_amd import speckle
     from d2ic_amd.mask2mesh import mask_to_mesh
     from d2ic_amd.mesh_assets import build_node_neighbor_tables
@@ -28,7 +28,7 @@ def _setup():
     from d2ic_amd.solver_global_cg import CGPlan
 
     m = speckle.rect_mask(H, W, 64, 64)
-    mesh = mask_to_mesh(m, 16)
+    mesh = mask_to_mesh(m, e)
     fn = speckle.affine_field([[2e-3, 1e-3], [-1e-3, 3e-3]], [0.4, -0.3], 0.2, 80.0, W / 2, H / 2)
     ref, de = speckle.speckle_pair(H, W, 21, fn, noise_sigma=1.0)
     pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=m)
@@ -92,11 +92,13 @@ def _exx_np(u, X, idx, deg):
     return out
 
 
-def test_hvp_matches_gradient_differences(gpu):
+@pytest.mark.parametrize("e", [16, 32])
+def test_hvp_matches_gradient_differences(gpu, e):
+    """e = 32: elements of 1024 pixels, whose tiles are 16-column strips."""
     from d2ic_amd.adjoint import Linearization
 
-    print("[case] adjoint hvp", flush=True)
-    m, mesh, ref, de, plan, oa, _, _ = _setup()
+    print(f"[case] adjoint hvp e={e}", flush=True)
+    m, mesh, ref, de, plan, oa, _, _ = _setup(e)
     t = gpu
     u0 = np.zeros((plan.n_nodes, 2))
     u = _u64(ref, de, oa, u0)
