@@ -146,7 +146,7 @@ struct PlanEntry {
 };
 
 constexpr int kMaxBlocks = 1024;
-constexpr int kArriveStride = 32;  // one 128-byte line per block's release word
+constexpr int kArriveStride = 32;  // one 128-byte line per block's arrival / release word
 // release word: step + 1 (mod 2^30) in the low bits, abort and session-stop flags above
 constexpr unsigned kReleaseAbort = 0x80000000u;
 constexpr unsigned kReleaseStop = 0x40000000u;
@@ -179,10 +179,7 @@ struct Workspace {
     int bad[3][kMaxProblems];
     PlanEntry plan[kMaxProblems];  // first step's plan (written by k_init)
     int n_tiles_active, n_chunks_active, any_active, pad0;
-    // last step each block finished: block b's word is arrive[b], packed so
-    // that the master's poll reads 8 lines, not one line per block (1 %
-    // per iteration); the rest of the array only keeps release[] where it was
-    unsigned arrive[kMaxBlocks * kArriveStride];
+    unsigned arrive[kMaxBlocks * kArriveStride];   // last step each block finished (one line per block)
     unsigned release[kMaxBlocks * kArriveStride];  // last step every block finished (one line per block)
     unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
     int stop;                                      // served mode: every block leaves after this barrier
@@ -1285,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             // used before the block arrived.)
             const bool wrote = blockIdx.x == 0 || (isA && nT > 0) || ci0 < ci1;
             if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&ws->arrive[blockIdx.x], (unsigned)(step + 1), __ATOMIC_RELAXED,
+            __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             s_ok = 1;
             s_stop = 0;
@@ -1307,7 +1304,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const int b = b0 + j * kWave + lane;
-                            v[j] = b < nb ? __hip_atomic_load(&ws->arrive[b], __ATOMIC_RELAXED,
+                            v[j] = b < nb ? __hip_atomic_load(&ws->arrive[b * kArriveStride], __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_AGENT)
                                           : (unsigned)(step + 1);
                         }
@@ -1518,50 +1515,44 @@ __global__ void k_heads(const unsigned long long* __restrict__ k, int64_t n, uin
     if (s < n) head[s] = (s == 0 || k[s] != k[s - 1]) ? 1 : 0;
 }
 
-// Elements of more than 256 pixels are regrouped before tiling: ns =
-// ceil(w / 16) column strips of balanced width sw = ceil(w / ns), each cut
-// into blocks of floor(256 / sw) whole rows, so a tile is one block (<= 256
-// pixels, <= 16 columns).  Its staged image box, (sw + 5) x (rows + 5) plus
-// the displacement range, then fits the wave's 512-float LDS slot for every
-// element shape; row-major 256-pixel runs of a 21..32-pixel-wide element
-// (e.g. 8 rows x 32 columns, a 37-38 x 13-14 box) often do not, and their
-// samples fall back to global memory.  Elements of <= 256 pixels (the 16 px
-// elements of the benchmark) keep one tile.  One thread per element group;
-// key = (group << 20) | (strip << 10) | block over the group's current
-// (row-major) order, so a stable sort keeps the rows of a block in order.
+// Elements wider than kStripMinW pixels (and of more than one tile) are
+// regrouped before tiling into ns = ceil(w / 16) column strips of balanced
+// width ceil(w / ns), relative to the element's first pixel column; the
+// 256-pixel runs of a strip then span <= 16 columns, and their staged image
+// box ((columns + 5) x (rows + 5) plus the displacement range) fits the
+// wave's 512-float LDS slot.  Row-major runs of a 32-pixel-wide element (8
+// rows x 32 columns, a 37-38 x 13-14 box) miss it on ~6 % of tiles, whose
+// samples then fall back to global memory.  Narrower elements keep row-major
+// runs: cutting them into strips or whole-row blocks adds partly filled tiles,
+// which cost more than the few boxes that miss the slot (measured).  One
+// thread per element group; key = (group << 16) | strip over the group's
+// current (row-major) order, so a stable sort keeps rows in order.
+constexpr int kStripMinW = 24;
 __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, int64_t np,
                                const int32_t* __restrict__ perm, const float* __restrict__ coords,
                                unsigned long long* __restrict__ key2, int* __restrict__ any_split) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int64_t s0 = gstart[g], s1 = (g + 1 < ng) ? gstart[g + 1] : np;
-    int xmin = 0, ymin = 0, sw = 1, rpt = 1;
-    const bool split = s1 - s0 > kTile;
-    if (split) {
+    int xmin = 0, sw = 1;
+    bool split = false;
+    if (s1 - s0 > kTile) {
         int xmax = INT32_MIN;
-        xmin = ymin = INT32_MAX;
+        xmin = INT32_MAX;
         for (int64_t s = s0; s < s1; ++s) {
-            const int64_t p = perm[s];
-            const int c = (int)floorf(coords[2 * p]), r = (int)floorf(coords[2 * p + 1]);
+            const int c = (int)floorf(coords[2 * (int64_t)perm[s]]);
             xmin = min(xmin, c);
             xmax = max(xmax, c);
-            ymin = min(ymin, r);
         }
         const int w = xmax - xmin + 1;
+        split = w > kStripMinW;
         const int ns = (w + 15) / 16;
         sw = (w + ns - 1) / ns;
-        rpt = kTile / sw;
     }
     for (int64_t s = s0; s < s1; ++s) {
-        unsigned long long sub = 0ull;
-        if (split) {
-            const int64_t p = perm[s];
-            const int c = (int)floorf(coords[2 * p]), r = (int)floorf(coords[2 * p + 1]);
-            const unsigned strip = (unsigned)min((c - xmin) / sw, 1023);
-            const unsigned blk = (unsigned)min((r - ymin) / rpt, 1023);
-            sub = ((unsigned long long)strip << 10) | blk;
-        }
-        key2[s] = ((unsigned long long)g << 20) | sub;
+        unsigned strip = 0;
+        if (split) strip = (unsigned)min(((int)floorf(coords[2 * (int64_t)perm[s]]) - xmin) / sw, 65535);
+        key2[s] = ((unsigned long long)g << 16) | strip;
     }
     if (split) atomicOr(any_split, 1);
 }
@@ -1924,7 +1915,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PH(hipMemcpyAsync(&ng, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         PH(hipStreamSynchronize(st));
         {
-            // elements of more than one tile: regroup by (element, strip, row block)
+            // wide elements: regroup by (element, column strip)
             Tmp any;
             PH(any.alloc(sizeof(int), st));
             PH(hipMemsetAsync(any.p, 0, sizeof(int), st));

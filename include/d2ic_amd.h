@@ -77,8 +77,8 @@ int d2ic_node_pixel_table(const int32_t* pixel_nodes, const float* pixel_shapeN,
 typedef struct d2ic_cg_plan d2ic_cg_plan;
 
 /* Build the solver's internal layout from PixelAssets arrays: pixels grouped
- * by element into tiles of <= 256 pixels (structure-of-arrays; elements of
- * more than 256 pixels in blocks of whole rows of <= 16-column strips), the
+ * by element into tiles of <= 256 pixels (structure-of-arrays; elements wider
+ * than 24 pixels by column strips of <= 16 pixels), the
  * node->tile gather table, and the spring-regularisation stencil
  * (node_neighbor_* and node_reg_weight, solver_global_cg.py:261-286).
  * pixel_elements may be NULL (elements are then identified by their node tuple). */
