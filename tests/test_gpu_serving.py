@@ -40,7 +40,10 @@ def _alone(plan, de, reg, max_iter, tol):
     return u.cpu().numpy(), hist.cpu().numpy(), n, p
 
 
-CASES = [(96, 64, 16, 1, 0.3, -0.2, 0.0), (160, 128, 8, 2, -0.4, 0.6, 20.0), (128, 96, 16, 3, 0.8, 0.1, 0.0)]
+# (H, ROI side, element px, seed, ux, uy, hole radius); the 32 px case has
+# elements of 1024 pixels, tiled as 16-column strips
+CASES = [(96, 64, 16, 1, 0.3, -0.2, 0.0), (160, 128, 8, 2, -0.4, 0.6, 20.0), (128, 96, 16, 3, 0.8, 0.1, 0.0),
+         (160, 128, 32, 4, 0.2, 0.5, 0.0)]
 
 
 def test_served_equals_alone(gpu):
@@ -48,7 +51,7 @@ def test_served_equals_alone(gpu):
     from d2ic_amd.serving import ServeSession
 
     t = _lib.torch()
-    print("[case] serve: 3 problems, 2 slots", flush=True)
+    print("[case] serve: 4 problems, 2 slots", flush=True)
     probs = [_plan(*c) for c in CASES]
     ref = [_alone(plan, de, 0.1, 60, 1e-4) for plan, de in probs]
     sess = ServeSession(slots=2, idle_ms=2.0)

@@ -112,6 +112,7 @@ def test_batch_equals_alone(gpu):
     print("[case] batch vs alone", flush=True)
     cases = [structured_case(H=96, W=96, side=64, e=16, seed=s, ux=0.2 * s, uy=-0.1 * s) for s in range(3)]
     cases.append(structured_case(H=128, W=128, side=96, e=8, seed=7, ux=-0.4, uy=0.3))
+    cases.append(structured_case(H=160, W=160, side=128, e=32, seed=9, ux=0.3, uy=0.2))  # strip-split tiles
     plans, imgs, us, alone = [], [], [], []
     for (m, mesh, ref, de) in cases:
         pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=m)
