@@ -111,7 +111,13 @@ def test_dic_run_discrepancy_output(gpu):
     mo, rmso = O.discrepancy_map(np.asarray(ref), np.asarray(de), u, _np_assets(pd))
     np.testing.assert_array_equal(res.pixel_maps["discrepancy_ref"].cpu().numpy(), mo)
     np.testing.assert_allclose(res.diagnostics.info["discrepancy_rms"], rmso, rtol=1e-6)
-    assert res.diagnostics.info["discrepancy"] == "discrepancy_ref"
+    info = res.diagnostics.info
+    assert info["discrepancy"] == "discrepancy_ref"
+    # the rest of the reference's diagnostics (dic_mesh_based.py:100-155)
+    assert info["stage"] == "mesh_based" and info["strain"] == "green_lagrange"
+    assert isinstance(info["n_iters"], int) and 0 < info["n_iters"] <= 50
+    assert info["history"] == "attached"  # save_history defaults to True
+    assert tuple(res.strain.shape) == (mesh.nodes_xy.shape[0], 3)
     # the solved displacement explains most of the residual at u = 0
     _, rms0 = O.discrepancy_map(np.asarray(ref), np.asarray(de), np.zeros_like(u), _np_assets(pd))
     print(f"rms at the solution {rmso:.3f}, at u = 0 {rms0:.3f}")
