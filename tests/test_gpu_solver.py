@@ -138,3 +138,33 @@ def test_batch_equals_alone(gpu):
     for k in range(P):
         assert int(cnts[k][0]) == alone[k][1]
         assert np.array_equal(us[k].cpu().numpy(), alone[k][0])
+
+
+@pytest.mark.parametrize("interp", ["linear", "cubic"])
+def test_cg_non_square_image_tracks_oracle(gpu, interp):
+    """An image wider than tall (80 x 176) with the ROI reaching columns past
+    H - 1: the reference's bilinear sampler clips x by H - 1 and y by W - 1
+    (solver_global_cg.py:495-500), which only shows on non-square images.
+    Same bars as the square case: alpha sequence exact, J / |grad| histories
+    and iterates against the FP64 oracle."""
+    from d2ic_amd import speckle
+    from d2ic_amd.mask2mesh import mask_to_mesh
+
+    print(f"[case] cg non-square 80x176 interp={interp}", flush=True)
+    H, W = 80, 176
+    m = np.zeros((H, W), bool)
+    m[8:72, 8:168] = True
+    mesh = mask_to_mesh(m, 16)
+    ref, de = speckle.speckle_pair(H, W, 4, speckle.rigid(0.3, -0.25), noise_sigma=1.0)
+    dic, res = _solve_gpu(mesh, ref, de, m, interp, max_iter=8, tol=1e-9, reg=0.1)
+    oa = O.pixel_assets(mesh.nodes_xy, mesh.elements, ref.shape, roi_mask=m)
+    assert float(np.asarray(oa["pixel_coords_ref"])[:, 0].max()) > H  # samples beyond column H - 1
+    u0 = np.zeros((mesh.nodes_xy.shape[0], 2), np.float32)
+    u64, h64, n64 = O.cg_solve(u0, ref, de, oa, alpha_reg=0.1, max_iter=8, tol=1e-9, interp=interp,
+                               dtype=np.float64)
+    hg = res.history.cpu().numpy()
+    assert res.diagnostics.info["n_iters"] == n64 == 8
+    np.testing.assert_array_equal(hg[:, 2], h64[:, 2].astype(np.float32))
+    np.testing.assert_allclose(hg[:, 0], h64[:, 0], rtol=2e-5)
+    np.testing.assert_allclose(hg[:, 1], h64[:, 1], rtol=2e-4)
+    np.testing.assert_allclose(res.u_nodal.cpu().numpy(), u64, atol=2e-5)
