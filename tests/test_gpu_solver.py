@@ -168,3 +168,36 @@ def test_cg_non_square_image_tracks_oracle(gpu, interp):
     np.testing.assert_allclose(hg[:, 0], h64[:, 0], rtol=2e-5)
     np.testing.assert_allclose(hg[:, 1], h64[:, 1], rtol=2e-4)
     np.testing.assert_allclose(res.u_nodal.cpu().numpy(), u64, atol=2e-5)
+
+
+def test_cg_empty_pixel_set_is_spring_only(gpu):
+    """No pixel passes the ROI filter (pixel_assets.py:62-73 with an all-false
+    roi_mask): J is the spring energy alone and J_img = 0 / max(0, 1)
+    (solver_global_cg.py:237-258); the iterates from a rough u0 still track the
+    FP64 oracle."""
+    from d2ic_amd import DICMeshBased, GlobalCGSolver, MeshDICConfig, make_mesh_assets, speckle
+    from d2ic_amd.mask2mesh import mask_to_mesh
+    from dataclasses import replace
+
+    print("[case] cg with an empty pixel set", flush=True)
+    t = gpu
+    H = W = 96
+    mesh = mask_to_mesh(speckle.rect_mask(H, W, 64, 64), 16)
+    none = np.zeros((H, W), bool)
+    ref, de = speckle.speckle_pair(H, W, 3, speckle.rigid(0.3, 0.1), noise_sigma=1.0)
+    u0 = np.random.default_rng(0).normal(scale=0.2, size=(mesh.nodes_xy.shape[0], 2)).astype(np.float32)
+    dic = DICMeshBased(mesh, GlobalCGSolver(interpolation="cubic"),
+                       MeshDICConfig(max_iters=8, tol=1e-9, reg_strength=0.1, save_history=True))
+    dic.prepare(ref, replace(make_mesh_assets(mesh), roi_mask=none))
+    assert int(dic.state.assets.pixel_data.pixel_coords_ref.shape[0]) == 0
+    dic.set_initial_guess(t.tensor(u0, device="cuda"))
+    res = dic.run(de)
+    oa = O.pixel_assets(mesh.nodes_xy, mesh.elements, (H, W), roi_mask=none)
+    u64, h64, n64 = O.cg_solve(u0, ref, de, oa, alpha_reg=0.1, max_iter=8, tol=1e-9, interp="cubic",
+                               dtype=np.float64)
+    hg = res.history.cpu().numpy()
+    assert res.diagnostics.info["n_iters"] == n64 == 8
+    np.testing.assert_array_equal(hg[:, 2], h64[:, 2].astype(np.float32))
+    np.testing.assert_allclose(hg[:, 0], h64[:, 0], rtol=2e-5)
+    np.testing.assert_allclose(res.u_nodal.cpu().numpy(), u64, atol=2e-5)
+    assert np.abs(u64 - u0).max() > 0.05  # the springs moved the nodes
