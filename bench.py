@@ -490,6 +490,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-local", action="store_true", help="skip the local Gauss-Newton refinement leg")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the persistent grid size with CGPlan.autotune before the timed frames")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -535,6 +537,12 @@ def main() -> int:
         dic.set_initial_guess(None)
         dic.run(frames[0])
     torch.cuda.synchronize()
+
+    tune = None
+    if args.autotune:
+        tune = solver.autotune_grid(dic.state, frames[0])
+        log(f"[bench] autotuned grid: {tune['blocks']} blocks ({tune['ms']})")
+        torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
     kt, kms, passes, iters = [], [], [], []
@@ -611,6 +619,8 @@ def main() -> int:
             "image": [w["H"], w["W"]], "element_size_px": w["e"], "n_pixels": Np, "n_nodes": Nn,
             "interpolation": w["interp"], "max_iters": w["max_iters"], "tol": w["tol"], "reg_strength": w["reg"],
             "iterations_per_frame": iters, "pixel_passes_per_frame": passes,
+            "grid_blocks": tune["blocks"] if tune else "default (work-based, capped at one block per CU)",
+            "autotune_ms": {str(k): v for k, v in tune["ms"].items()} if tune else None,
         },
         "roofline": {
             "bound": "hbm",

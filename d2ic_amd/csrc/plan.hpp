@@ -51,6 +51,9 @@ struct d2ic_cg_plan {
     // start/end of the last persistent-kernel launch this plan led (plans[0])
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
     bool timed = false;
+    // persistent grid size for single-problem solves led by this plan (0: the
+    // work-based default), set by d2ic_cg_plan_set_grid (the autotuner)
+    int grid_blocks = 0;
 };
 
 namespace d2ic {

@@ -2031,6 +2031,13 @@ int d2ic_cg_plan_info(const d2ic_cg_plan* plan, int64_t* n_pixels, int64_t* n_no
     return D2IC_OK;
 }
 
+int d2ic_cg_plan_set_grid(d2ic_cg_plan* plan, int32_t blocks) {
+    D2IC_CHECK_ARG(plan != nullptr, "plan must not be NULL");
+    D2IC_CHECK_ARG(blocks >= 0 && blocks <= kMaxBlocks, "blocks must be in [0, %d]", kMaxBlocks);
+    plan->grid_blocks = blocks;
+    return D2IC_OK;
+}
+
 int d2ic_cg_kernel_ms(const d2ic_cg_plan* plan, float* ms) {
     D2IC_CHECK_ARG(plan != nullptr && ms != nullptr, "plan and ms must not be NULL");
     if (!plan->timed) {
@@ -2143,6 +2150,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     // split exact
     int64_t want = ceil_div(total_work, (int64_t)kWavesPerBlock * 4);
     want = std::max<int64_t>(8, ceil_div(want, 8) * 8);
+    if (P == 1 && plans[0]->grid_blocks > 0) want = ceil_div((int64_t)plans[0]->grid_blocks, 8) * 8;
     int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kMaxBlocks), want);
     int max_steps = 8 + 2 * (2 + max_it * 12);
     ServeHost* no_sh = nullptr;

@@ -78,6 +78,7 @@ class CGPlan:
                   md, ctypes.byref(h), _lib.stream())
         self.handle = h
         self._ref_key = None
+        self.grid_blocks = 0
         self._keep = []  # the plan copied what it needs
 
     def info(self):
@@ -91,6 +92,46 @@ class CGPlan:
         ms = ctypes.c_float(0.0)
         _lib.call("d2ic_cg_kernel_ms", self.handle, ctypes.byref(ms))
         return float(ms.value)
+
+    def set_grid(self, blocks: int) -> None:
+        """Persistent grid size (blocks) for single solves on this plan; 0
+        restores the work-based default.  Only speed depends on it."""
+        _lib.call("d2ic_cg_plan_set_grid", self.handle, int(blocks))
+        self.grid_blocks = int(blocks)
+
+    def autotune(self, def_image, alpha_reg: float = 0.1, max_iter: int = 8, candidates=None,
+                 repeats: int = 3) -> dict:
+        """Choose the persistent grid size by timing short solves.
+
+        Each candidate grid (blocks, multiples of 8 up to one block per CU)
+        runs ``repeats`` solves of ``max_iter`` iterations from u = 0 with
+        tol = 0, so every candidate does the same passes; the kernel time is
+        read from the HIP events around the launch and the fastest median
+        wins and is kept (``set_grid``).  The reference image must be set.
+        Returns ``{"blocks": best, "ms": {blocks: median ms}}``."""
+        t = _lib.torch()
+        if self._ref_key is None:
+            raise RuntimeError("set_reference() must be called before autotune()")
+        g = _lib.to_dev(def_image, t.float32)
+        if candidates is None:
+            cu = int(t.cuda.get_device_properties(_lib.device()).multi_processor_count)
+            candidates = sorted({max(8, (cu * k // 8 + 7) // 8 * 8) for k in (2, 4, 6, 8)})
+        u = t.zeros((self.n_nodes, 2), dtype=t.float32, device=_lib.device())
+        cnt = t.zeros((2,), dtype=t.int32, device=_lib.device())
+        times = {}
+        for b in candidates:
+            self.set_grid(int(b))
+            ms = []
+            for _ in range(int(repeats) + 1):  # the first run warms caches and is dropped
+                u.zero_()
+                _lib.call("d2ic_cg_solve", self.handle, _lib.ptr(g), int(g.shape[0]), int(g.shape[1]), _lib.ptr(u),
+                          float(alpha_reg), int(max_iter), 0.0, None, _lib.ptr(cnt), _lib.ptr(cnt) + 4,
+                          _lib.stream())
+                ms.append(self.kernel_ms())
+            times[int(b)] = float(np.median(ms[1:]))
+        best = min(times, key=times.get)
+        self.set_grid(best)
+        return {"blocks": best, "ms": times}
 
     def set_reference(self, ref_image, interpolation: str):
         t = _lib.torch()
@@ -165,6 +206,16 @@ class GlobalCGSolver(SolverBase):
     @property
     def plan(self) -> CGPlan | None:
         return self._plan
+
+    def autotune_grid(self, state: Any, def_image: Array, **kwargs) -> dict:
+        """Pick the persistent grid size for this mesh from short timed solves
+        on ``def_image`` (``CGPlan.autotune``); later solves use it.  No
+        reference counterpart: launch tuning only, results are unchanged."""
+        if not self._compiled or self._plan is None:
+            raise RuntimeError("GlobalCGSolver.compile() must be called before autotune_grid().")
+        self._plan.set_reference(state.ref_image, self._interpolation)
+        kwargs.setdefault("alpha_reg", float(state.config.reg_strength))
+        return self._plan.autotune(def_image, **kwargs)
 
     def close(self) -> None:
         """Release the device plan."""

@@ -113,6 +113,13 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
                         const int32_t* max_iter, const float* tol, float* const* hist, int32_t* const* n_iters,
                         int32_t* const* pass_count, void* stream);
 
+/* Grid size (blocks, rounded up to a multiple of 8 and capped by what can be
+ * co-resident) of the persistent kernel for single-problem solves with this
+ * plan; 0 restores the work-based default.  Results do not depend on it
+ * (every reduction is order-independent); only speed does.  No reference
+ * counterpart (launch tuning, used by CGPlan.autotune). */
+int d2ic_cg_plan_set_grid(d2ic_cg_plan* plan, int32_t blocks);
+
 /* Duration in ms of the last persistent CG kernel launch that this plan led
  * (first plan of the batch), from HIP events recorded around that launch on
  * its stream; waits for the launch to finish.  No reference counterpart

@@ -1,0 +1,52 @@
+"""Persistent grid size (d2ic_cg_plan_set_grid) and the grid autotuner.
+
+Every reduction of the persistent NLCG kernel is an order-independent
+fixed-point sum, so the solve is bitwise the same whatever the grid: u, the
+[J, |grad|, alpha] history and the iteration count are compared exactly for
+the work-based default grid and forced grids of 8, 64 and 256 blocks.  The
+autotuner must return one of its candidates, keep it on the plan, and leave
+results unchanged."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_solve_is_bitwise_identical_across_grids(gpu):
+    from d2ic_amd import DICMeshBased, GlobalCGSolver, MeshDICConfig, mask_to_mesh_assets, speckle
+
+    H = W = 512
+    print("[case] grid sizes 0/8/64/256 on 512^2 Q4-16, then autotune", flush=True)
+    m = speckle.plate_with_hole_mask(H, W, 440, 70.0)
+    mesh, assets = mask_to_mesh_assets(m, 16)
+    ref, de = speckle.speckle_pair(H, W, 5, speckle.rigid(0.45, -0.3), noise_sigma=0.5)
+    solver = GlobalCGSolver(interpolation="cubic")
+    dic = DICMeshBased(mesh, solver, MeshDICConfig(max_iters=60, tol=1e-5, reg_strength=0.1, save_history=True))
+    dic.prepare(ref, assets)
+    out = {}
+    for b in (0, 8, 64, 256):
+        solver.plan.set_grid(b)
+        dic.set_initial_guess(None)
+        res = dic.run(de)
+        out[b] = (res.u_nodal.cpu().numpy(), res.history.cpu().numpy(), res.diagnostics.info["n_iters"],
+                  solver.last_pass_count)
+        print(f"grid {b}: iters {out[b][2]} passes {out[b][3]} kernel {solver.plan.kernel_ms():.3f} ms")
+    u0, h0, n0, p0 = out[0]
+    assert 0 < n0 <= 60
+    for b in (8, 64, 256):
+        u, h, n, p = out[b]
+        assert n == n0 and p == p0
+        np.testing.assert_array_equal(u, u0)
+        np.testing.assert_array_equal(h, h0)
+
+    tune = solver.autotune_grid(dic.state, de, candidates=(16, 64, 128), max_iter=6, repeats=2)
+    print(f"autotune {tune}")
+    assert tune["blocks"] in (16, 64, 128) and set(tune["ms"]) == {16, 64, 128}
+    assert all(v > 0.0 for v in tune["ms"].values())
+    assert solver.plan.grid_blocks == tune["blocks"]
+    dic.set_initial_guess(None)
+    res = dic.run(de)
+    np.testing.assert_array_equal(res.u_nodal.cpu().numpy(), u0)
+    np.testing.assert_array_equal(res.history.cpu().numpy(), h0)
