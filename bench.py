@@ -101,6 +101,45 @@ def _random_motion(rng, H, max_grad=5e-3, max_amp=3.0):
     return A, b, amp, wl
 
 
+def stages_leg(mesh, assets, mask, ref, de, reps=5, win=41, search=4):
+    """Stages around the solve on the benchmark workload, each timed as the
+    median of ``reps`` calls after one untimed call (device-synchronised wall
+    time): pixel assets from the mesh (device kernels plus the host spring
+    graph), ZNCC at every element centre, IDW of the centre shifts to the
+    nodes, Green-Lagrange LSQ strain at the nodes.  ZNCC is priced as 5 FP32
+    flops per window pixel per candidate (sums of g, g^2, f g) against the
+    FP32 vector peak."""
+    import torch
+
+    from d2ic_amd.dic_init_motion import project_centers_to_nodes_idw
+    from d2ic_amd.pixel_assets import build_pixel_assets
+    from d2ic_amd.solver_translation_zncc import match_centers
+    from d2ic_amd.strain import strain_voigt
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3, out
+
+    c = np.asarray(assets.element_centers_xy, dtype=np.float32)
+    ms_pa, pa = timed(lambda: build_pixel_assets(mesh, ref, 1.0, roi_mask=mask))
+    ms_zn, (u_c, _) = timed(lambda: match_centers(c, ref, de, win=win, search=search))
+    ms_idw, u_n = timed(lambda: project_centers_to_nodes_idw(mesh.nodes_xy, c, u_c))
+    nodes = torch.as_tensor(np.asarray(mesh.nodes_xy), dtype=torch.float32, device="cuda")
+    ms_st, _ = timed(lambda: strain_voigt(u_n, nodes, assets.node_neighbor_index, assets.node_neighbor_degree))
+    zn_flop = 5.0 * (win * win) * (2 * search + 1) ** 2 * len(c)
+    return dict(pixel_assets_ms=ms_pa, n_pixels=int(pa.pixel_coords_ref.shape[0]),
+                zncc_ms=ms_zn, zncc_centres=int(len(c)), zncc_win=win, zncc_search=search,
+                zncc_tflops=zn_flop / (ms_zn * 1e-3) / 1e12, zncc_peak_tflops=157.3,
+                idw_ms=ms_idw, strain_ms=ms_st, n_nodes=int(nodes.shape[0]))
+
+
 def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1):
     """64 independent frame-pair solves of uneven size: one persistent launch
     (d2ic_cg_solve_batch) vs one launch per problem (d2ic_cg_solve)."""
@@ -578,6 +617,12 @@ def main() -> int:
         if u_prev is not None:
             local["discrepancy"] = discrepancy_leg(dic, ref, frames[K - 1], u_prev, Np)
 
+    stages = None
+    if not args.no_local:
+        log("[bench] stages leg: pixel assets, ZNCC at every element centre, IDW, strain")
+        stages = stages_leg(mesh, dic.state.assets, mask, ref, frames[0])
+        log(f"[bench] stages: {stages}")
+
     batch = serving = None
     if not args.no_batch:
         log("[bench] batched leg: 64 uneven problems")
@@ -640,6 +685,7 @@ def main() -> int:
         "serving": serving,
         "differentiation": diff,
         "local_refine": local,
+        "stages": stages,
     }
     print(json.dumps(line), flush=True)
     # release device objects before interpreter teardown (profilers finalise then)

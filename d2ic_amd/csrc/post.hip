@@ -15,11 +15,12 @@ namespace d2ic {
 
 // ---------------------------------------------------------------------------
 // ZNCC: one block per centre.  The normalised template and the search patch
-// are staged in LDS; each thread scores windows (ky, kx) in row-major order;
+// are staged in LDS; kZnccSplit threads score each window (ky, kx);
 // the block takes the first maximum (a NaN counts as the maximum, like
 // jnp.argmax).  Statistics are accumulated in FP64.
 // ---------------------------------------------------------------------------
 constexpr int kZnccBlock = 256;
+constexpr int kZnccSplit = 3;
 
 __device__ __forceinline__ bool better(float a, int ia, float b, int ib) {
     // is (a, ia) preferred over (b, ib) under "first max, NaN wins"?
@@ -90,32 +91,66 @@ __global__ __launch_bounds__(kZnccBlock) void k_zncc(const float* __restrict__ c
     for (int k = threadIdx.x; k < n; k += blockDim.x) tpl[k] = (tpl[k] - tmean) / tden;
     __syncthreads();
 
+    // each candidate window's rows are split over kZnccSplit consecutive
+    // threads; the three passes (mean, variance, correlation) combine the
+    // split's FP64 partials in a fixed order through LDS
     float best = 0.f;
     int bidx = 0x7fffffff;
     bool have = false;
-    for (int wi = threadIdx.x; wi < nwin * nwin; wi += blockDim.x) {
+    const int nc = nwin * nwin;
+    constexpr int cap = (kZnccBlock / kZnccSplit) * kZnccSplit;
+    const int rows_per = (win + kZnccSplit - 1) / kZnccSplit;
+    const int tid = threadIdx.x;
+    for (int b0 = 0; b0 < nc * kZnccSplit; b0 += cap) {
+        const int item = b0 + tid;
+        const bool act = tid < cap && item < nc * kZnccSplit;
+        const int wi = act ? item / kZnccSplit : 0, g = act ? item % kZnccSplit : 0;
+        const int lead = tid - g;  // thread holding the split's first partial
+        const int i0 = min(win, g * rows_per), i1 = min(win, i0 + rows_per);
         const int ky = wi / nwin, kx = wi % nwin;
         const float* base = patch + ky * P + kx;
         double sw = 0.0;
-        for (int i = 0; i < win; ++i)
-            for (int j = 0; j < win; ++j) sw += base[i * P + j];
-        const float m = (float)(sw / n);
+        if (act)
+            for (int i = i0; i < i1; ++i)
+                for (int j = 0; j < win; ++j) sw += base[i * P + j];
+        red[tid] = sw;
+        __syncthreads();
+        double tot = 0.0;
+        if (act)
+            for (int q = 0; q < kZnccSplit; ++q) tot += red[lead + q];
+        const float m = (float)(tot / n);
+        __syncthreads();
         double sv = 0.0;
-        for (int i = 0; i < win; ++i)
-            for (int j = 0; j < win; ++j) {
-                float d = base[i * P + j] - m;
-                sv += (double)(d * d);
-            }
-        const float den = (float)sqrt(sv / n) + 1e-12f;
+        if (act)
+            for (int i = i0; i < i1; ++i)
+                for (int j = 0; j < win; ++j) {
+                    float d = base[i * P + j] - m;
+                    sv += (double)(d * d);
+                }
+        red[tid] = sv;
+        __syncthreads();
+        tot = 0.0;
+        if (act)
+            for (int q = 0; q < kZnccSplit; ++q) tot += red[lead + q];
+        const float den = (float)sqrt(tot / n) + 1e-12f;
+        __syncthreads();
         double acc = 0.0;
-        for (int i = 0; i < win; ++i)
-            for (int j = 0; j < win; ++j) acc += (double)(((base[i * P + j] - m) / den) * tpl[i * win + j]);
-        const float sc = (float)(acc / n);
-        if (!have || better(sc, wi, best, bidx)) {
-            best = sc;
-            bidx = wi;
-            have = true;
+        if (act)
+            for (int i = i0; i < i1; ++i)
+                for (int j = 0; j < win; ++j) acc += (double)(((base[i * P + j] - m) / den) * tpl[i * win + j]);
+        red[tid] = acc;
+        __syncthreads();
+        if (act && g == 0) {
+            tot = 0.0;
+            for (int q = 0; q < kZnccSplit; ++q) tot += red[lead + q];
+            const float sc = (float)(tot / n);
+            if (!have || better(sc, wi, best, bidx)) {
+                best = sc;
+                bidx = wi;
+                have = true;
+            }
         }
+        __syncthreads();
     }
     s_best[threadIdx.x] = best;
     s_bidx[threadIdx.x] = have ? bidx : 0x7fffffff;
@@ -142,28 +177,38 @@ __global__ __launch_bounds__(kZnccBlock) void k_zncc(const float* __restrict__ c
 }
 
 // ---------------------------------------------------------------------------
-// IDW (power 2) with exact-hit copy; centres staged through LDS in tiles.
+// IDW (power 2) with exact-hit copy.  A block serves 64 nodes (one per
+// lane); its kIdwWaves waves take interleaved 64-centre tiles, each staged in
+// the wave's own LDS slot, and the waves' FP64 partial sums are added in wave
+// order.  The copied centre is the first one within 1e-6 (smallest index):
+// each wave stops at its own first hit and the smallest hit index wins.
 // ---------------------------------------------------------------------------
-constexpr int kIdwTile = 256;
+constexpr int kIdwWaves = 8;
 
-__global__ __launch_bounds__(256) void k_idw(const float* __restrict__ nodes, int64_t nn,
-                                             const float* __restrict__ centers, const float* __restrict__ disp,
-                                             int64_t nc, float* __restrict__ out) {
-    __shared__ float4 sc[kIdwTile];
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64 * kIdwWaves) void k_idw(const float* __restrict__ nodes, int64_t nn,
+                                                        const float* __restrict__ centers,
+                                                        const float* __restrict__ disp, int64_t nc,
+                                                        float* __restrict__ out) {
+    __shared__ float4 sc[kIdwWaves][64];
+    __shared__ double part[3][kIdwWaves][64];
+    __shared__ int64_t hit[kIdwWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t n = (int64_t)blockIdx.x * 64 + lane;
     const bool valid = n < nn;
     const float nx = valid ? nodes[2 * n] : 0.f, ny = valid ? nodes[2 * n + 1] : 0.f;
     double ws = 0.0, ax = 0.0, ay = 0.0;
     int64_t close = -1;
-    for (int64_t c0 = 0; c0 < nc; c0 += kIdwTile) {
-        __syncthreads();
-        int64_t ci = c0 + threadIdx.x;
-        if (ci < nc) sc[threadIdx.x] = make_float4(centers[2 * ci], centers[2 * ci + 1], disp[2 * ci], disp[2 * ci + 1]);
-        __syncthreads();
-        const int m = (int)min<int64_t>(kIdwTile, nc - c0);
+    for (int64_t c0 = (int64_t)w * 64; c0 < nc; c0 += 64 * kIdwWaves) {
+        // LDS operations of one wave complete in order: the previous tile's
+        // reads are done before these writes, and these writes before the reads
+        const int64_t ci = c0 + lane;
+        __builtin_amdgcn_wave_barrier();
+        if (ci < nc) sc[w][lane] = make_float4(centers[2 * ci], centers[2 * ci + 1], disp[2 * ci], disp[2 * ci + 1]);
+        __builtin_amdgcn_wave_barrier();
+        const int m = (int)min<int64_t>(64, nc - c0);
         if (valid && close < 0) {
             for (int k = 0; k < m; ++k) {
-                const float4 q = sc[k];
+                const float4 q = sc[w][k];
                 const float dx = q.x - nx, dy = q.y - ny;
                 const float dist = sqrtf(dx * dx + dy * dy);
                 if (dist < 1e-6f) {
@@ -171,14 +216,28 @@ __global__ __launch_bounds__(256) void k_idw(const float* __restrict__ nodes, in
                     break;
                 }
                 const float dd = dist + 1e-12f;
-                const float w = 1.0f / (dd * dd);
-                ws += w;
-                ax += (double)w * q.z;
-                ay += (double)w * q.w;
+                const float wgt = 1.0f / (dd * dd);
+                ws += wgt;
+                ax += (double)wgt * q.z;
+                ay += (double)wgt * q.w;
             }
         }
     }
-    if (!valid) return;
+    part[0][w][lane] = ws;
+    part[1][w][lane] = ax;
+    part[2][w][lane] = ay;
+    hit[w][lane] = close;
+    __syncthreads();
+    if (w != 0 || !valid) return;
+    ws = ax = ay = 0.0;
+    close = -1;
+    for (int q = 0; q < kIdwWaves; ++q) {
+        ws += part[0][q][lane];
+        ax += part[1][q][lane];
+        ay += part[2][q][lane];
+        const int64_t h = hit[q][lane];
+        if (h >= 0 && (close < 0 || h < close)) close = h;
+    }
     if (close >= 0) {
         out[2 * n] = disp[2 * close];
         out[2 * n + 1] = disp[2 * close + 1];
@@ -471,7 +530,7 @@ int d2ic_idw(const float* nodes, int64_t n_nodes, const float* centers, const fl
         D2IC_HIP(hipMemsetAsync(u_out, 0, sizeof(float) * 2 * n_nodes, st));
         return D2IC_OK;
     }
-    k_idw<<<ceil_div(n_nodes, 256), 256, 0, st>>>(nodes, n_nodes, centers, disp, n_centers, u_out);
+    k_idw<<<ceil_div(n_nodes, 64), 64 * kIdwWaves, 0, st>>>(nodes, n_nodes, centers, disp, n_centers, u_out);
     D2IC_LAUNCH_CHECK();
     return D2IC_OK;
 }

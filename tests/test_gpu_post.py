@@ -30,16 +30,20 @@ def test_zncc_reference_fixture(gpu):
     np.testing.assert_allclose(s, so, rtol=1e-5)
 
 
-def test_zncc_out_of_bounds_and_speckle(gpu):
+@pytest.mark.parametrize("win, search", [(21, 6), (41, 4), (3, 9)])
+def test_zncc_out_of_bounds_and_speckle(gpu, win, search):
+    """Window rows are split over three threads per candidate; (21, 6) and
+    (3, 9) need several candidate batches per block, (41, 4) is the
+    initialisation used by the benchmark, (3, 9) gives each thread one row."""
     from d2ic_amd import speckle
     from d2ic_amd.solver_translation_zncc import match_centers
 
-    print("[case] zncc speckle", flush=True)
+    print(f"[case] zncc speckle win={win} search={search}", flush=True)
     ref, de = speckle.speckle_pair(128, 128, 5, speckle.rigid(3.2, -1.7), noise_sigma=1.0)
     rng = np.random.default_rng(1)
     centers = rng.uniform(0, 128, size=(64, 2)).astype(np.float32)
-    u, s = match_centers(centers, ref, de, win=21, search=6)
-    uo, so = O.zncc(centers, ref, de, 21, 6)
+    u, s = match_centers(centers, ref, de, win=win, search=search)
+    uo, so = O.zncc(centers, ref, de, win, search)
     u, s = u.cpu().numpy(), s.cpu().numpy()
     assert np.array_equal(np.isnan(s), np.isnan(so))
     assert np.array_equal(u, uo)
@@ -60,6 +64,27 @@ def test_idw_matches_oracle(gpu):
     ref = O.idw(nodes, centers, disp)
     np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-6)
     assert np.array_equal(out[10], disp[3])
+
+
+def test_idw_many_centres_first_hit(gpu):
+    """Centres are split over the waves of a block in interleaved 64-centre
+    tiles: sums vs the oracle, and the first (smallest-index) exact hit wins
+    when two centres sit on one node in tiles of different waves."""
+    from d2ic_amd.dic_init_motion import project_centers_to_nodes_idw
+
+    print("[case] idw 3000 nodes x 1500 centres", flush=True)
+    rng = np.random.default_rng(5)
+    nodes = rng.uniform(0, 500, size=(3000, 2)).astype(np.float32)
+    centers = rng.uniform(0, 500, size=(1500, 2)).astype(np.float32)
+    disp = rng.normal(size=(1500, 2)).astype(np.float32)
+    centers[700] = nodes[5]
+    centers[100] = nodes[5]
+    centers[1300] = nodes[7]
+    out = project_centers_to_nodes_idw(nodes, centers, disp).cpu().numpy()
+    ref = O.idw(nodes, centers, disp)
+    np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(out[5], disp[100]) and np.array_equal(ref[5], disp[100])
+    assert np.array_equal(out[7], disp[1300])
 
 
 def test_strain_reference_closed_form(gpu):
