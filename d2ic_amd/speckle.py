@@ -8,9 +8,11 @@ are scaled with the reference image's min/max: 255 (S - min S_ref) /
 (max S_ref - min S_ref).  A deformed image re-renders the spots at
 c_k + u(c_k) and adds N(0, noise_sigma^2) grey-level noise (independent seed).
 
-``render_spots`` here is the numpy renderer (fine up to ~1 Mpx); the device
-renderer ``d2ic_amd.device.render_speckle`` produces the same images on the
-GPU for the large workloads.
+``render_spots`` here is the numpy renderer (fine up to ~1 Mpx);
+``render_spots_device`` (the HIP renderer ``d2ic_render_speckle``) produces
+the same sums on the GPU for the large workloads, and
+``speckle_sequence_device`` builds whole sequences there (its noise comes from
+torch's device generator, so its noisy frames differ from ``speckle_pair``'s).
 """
 from __future__ import annotations
 
