@@ -27,78 +27,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = dict(H=2048, W=2048, side=1800, hole=300.0, e=16, seed=0, max_iters=200, tol=1e-3, reg=0.1,
-                interp="cubic")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def build_workload(n_frames: int):
-    from d2ic_amd import speckle
-
-    w = WORKLOAD
-    fns = [speckle.affine_sinusoid(t, w["W"], w["H"]) for t in range(1, n_frames + 1)]
-    ref, frames = speckle.speckle_sequence_device(w["H"], w["W"], w["seed"], fns, noise_sigma=1.0)
-    mask = speckle.plate_with_hole_mask(w["H"], w["W"], w["side"], w["hole"])
-    return ref, frames, mask
-
-
-def _zncc_init(mesh, assets, ref, de, search=4, win=41, max_centres=4096):
-    """ZNCC shifts at (a regular subsample of) the element centres, IDW to the
-    nodes: the initial guess shared by the plain and the batched/served paths."""
-    from d2ic_amd.dic_init_motion import project_centers_to_nodes_idw
-    from d2ic_amd.solver_translation_zncc import match_centers
-
-    c = np.asarray(assets.element_centers_xy, dtype=np.float32)
-    step = max(1, int(np.ceil(len(c) / max_centres)))
-    c = c[::step]
-    u, s = match_centers(c, ref, de, win=win, search=search)
-    return project_centers_to_nodes_idw(mesh.nodes_xy, c, u)
-
-
-def _problem(rng, H, e, roi, A, b, amp, wl, seed):
-    """One frame pair: speckle of the given size, ROI, Q4 mesh and plan."""
-    import torch
-
-    from d2ic_amd import speckle
-    from d2ic_amd.mask2mesh import mask_to_mesh
-    from d2ic_amd.mesh_assets import compute_element_centers
-    from d2ic_amd.pixel_assets import build_pixel_assets
-    from d2ic_amd.solver_global_cg import CGPlan
-
-    W = H
-    if roi == "hole":
-        mask = speckle.plate_with_hole_mask(H, W, int(0.88 * H), 0.15 * H)
-    else:
-        mask = speckle.rect_mask(H, W, int(0.8 * H), int(0.8 * H))
-    mesh = mask_to_mesh(mask, e)
-    fn = speckle.affine_field(A, b, amp, wl, W / 2.0, H / 2.0)
-    ref, frames = speckle.speckle_sequence_device(H, W, seed, [fn], noise_sigma=1.0)
-    pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=mask)
-    plan = CGPlan(pa)
-    plan.set_reference(ref, "cubic")
-
-    class _A:
-        element_centers_xy = compute_element_centers(mesh)
-
-    u0 = _zncc_init(mesh, _A, ref, frames[0])
-    torch.cuda.synchronize()
-    return dict(H=H, e=e, plan=plan, img=frames[0], u0=u0, np=plan.n_pixels, nn=plan.n_nodes, ref=ref)
-
-
-def _random_motion(rng, H, max_grad=5e-3, max_amp=3.0):
-    A = rng.uniform(-max_grad, max_grad, size=(2, 2)) / 2.0
-    half = H * 0.45
-    lin = np.abs(A).sum(axis=1).max() * half
-    amp = rng.uniform(0.2, 1.0)
-    bmax = max(0.0, max_amp - lin - amp)
-    b = rng.uniform(-bmax, bmax, size=2) / np.sqrt(2.0)
-    wl = float(H) * rng.uniform(0.8, 2.0)
-    # keep the sinusoid's gradient within max_grad as well
-    amp = min(amp, max_grad * wl / (2 * np.pi))
-    return A, b, amp, wl
 
 
 def stages_leg(mesh, assets, mask, ref, de, reps=5, win=41, search=4):
@@ -140,24 +72,38 @@ def stages_leg(mesh, assets, mask, ref, de, reps=5, win=41, search=4):
                 idw_ms=ms_idw, strain_ms=ms_st, n_nodes=int(nodes.shape[0]))
 
 
-def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1):
+def _oracle_accuracy(p, u_gpu, reg, max_iters, tol, interp="cubic"):
+    """max |u - u_oracle| on the nodes that pixels touch, for one problem of the
+    batch/serve mixes: the GPU result against the FP32 and FP64 oracle runs of
+    the same solve from the same initial guess."""
+    from oracle import oracle as O
+
+    ref = p["ref"].cpu().numpy()
+    img = p["img"].cpu().numpy()
+    oa = O.pixel_assets(p["mesh"].nodes_xy, p["mesh"].elements, ref.shape, roi_mask=p["mask"])
+    used = oa["node_pixel_degree"] > 0
+    u0 = p["u0"].cpu().numpy()
+    out = {}
+    for name, dt in (("fp32", np.float32), ("fp64", np.float64)):
+        u, _, n = O.cg_solve(u0, ref, img, oa, alpha_reg=reg, max_iter=max_iters, tol=tol, interp=interp, dtype=dt)
+        out[f"max_abs_du_vs_oracle_{name}_px"] = float(np.abs(u_gpu - u)[used].max())
+        out[f"oracle_{name}_iters"] = int(n)
+    return out
+
+
+def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1, accuracy_samples=3):
     """64 independent frame-pair solves of uneven size: one persistent launch
-    (d2ic_cg_solve_batch) vs one launch per problem (d2ic_cg_solve)."""
+    (d2ic_cg_solve_batch) vs one launch per problem (d2ic_cg_solve), plus the
+    accuracy of a few sampled problems (sides <= 1024) against the oracle."""
     import ctypes
 
     import torch
 
     from d2ic_amd import _lib
+    from d2ic_amd.workloads import batch_problems
 
-    rng = np.random.default_rng(seed)
-    sizes = rng.choice([512, 1024, 2048, 4096], size=n, p=[0.4, 0.3, 0.2, 0.1])
-    elems = rng.choice([8, 16, 32], size=n)
-    rois = rng.choice(["rect", "hole"], size=n)
     t0 = time.perf_counter()
-    probs = []
-    for i in range(n):
-        A, b, amp, wl = _random_motion(rng, int(sizes[i]))
-        probs.append(_problem(rng, int(sizes[i]), int(elems[i]), str(rois[i]), A, b, amp, wl, seed * 1000 + i))
+    probs = batch_problems(seed=seed, n=n)
     log(f"[batch] {n} problems prepared in {time.perf_counter() - t0:.1f} s; "
         f"{sum(p['np'] for p in probs) / 1e6:.1f} Mpx, {sum(p['nn'] for p in probs)} nodes")
     st = _lib.stream()
@@ -199,14 +145,25 @@ def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1):
     t_plain = time.perf_counter() - t0
     same = all(torch.equal(a, b) for a, b in zip(ub, up)) and itb == itp
     mpx_pass = sum(p["np"] * q for p, q in zip(probs, pb)) / 1e6
-    res = dict(n_problems=n, sizes={int(k): int((sizes == k).sum()) for k in (512, 1024, 2048, 4096)},
+    acc = []
+    if accuracy_samples > 0:
+        small = [i for i, p in enumerate(probs) if p["H"] <= 1024][:accuracy_samples]
+        for i in small:
+            a = _oracle_accuracy(probs[i], ub[i].cpu().numpy(), reg, max_iters, tol)
+            a.update(problem=i, side=probs[i]["H"], element_px=probs[i]["e"], roi=probs[i]["roi"],
+                     gpu_iters=itb[i])
+            acc.append(a)
+    sizes = [p["H"] for p in probs]
+    res = dict(n_problems=n, sizes={k: sizes.count(k) for k in (512, 1024, 2048, 4096)},
                mpx=sum(p["np"] for p in probs) / 1e6, batched_s=t_batch, plain_s=t_plain,
                batched_frames_per_s=n / t_batch, plain_frames_per_s=n / t_plain, speedup=t_plain / t_batch,
                bitwise_equal=bool(same), iterations_mean=float(np.mean(itb)), pixel_passes_total=int(sum(pb)),
-               batched_mpx_pass_per_s=mpx_pass / t_batch, init="ZNCC (win 41, search 4) at <=4096 element centres "
-                                                            "+ IDW, shared by both paths")
+               batched_mpx_pass_per_s=mpx_pass / t_batch,
+               accuracy=dict(note="batched results are bitwise the plain ones (same accuracy); sampled problems "
+                                  "against the oracle from the same initial guess", samples=acc),
+               init="ZNCC (win 41, search 4) at <=4096 element centres + IDW, shared by both paths")
     log(f"[batch] batched {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, speedup {t_plain / t_batch:.2f}, "
-        f"bitwise equal {same}")
+        f"bitwise equal {same}; accuracy {acc}")
     for p in probs:
         p["plan"].close()
     return res
@@ -215,60 +172,37 @@ def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1):
 def serving_leg(seed=11, n_streams=16, rate=200.0, seconds=1.0, max_iters=200, tol=1e-3, reg=0.1):
     """16 camera streams, Poisson arrivals (aggregate `rate` frame pairs/s for
     `seconds`), ROI side log-uniform in [256, 4096], Q4 elements of 8..32 px,
-    smooth affine+sinusoid motion; served (continuous batching) vs plain FIFO."""
+    smooth affine+sinusoid motion; served (continuous batching) vs plain FIFO.
+
+    Steady state: the number of requests waiting (arrived, not finished) is
+    sampled through the run; when it still grows at the end of the arrival
+    window the server is not keeping up, and the sustained completion rate is
+    what the leg reports (latency percentiles then measure queue growth)."""
     import ctypes
 
     import torch
 
-    from d2ic_amd import _lib, speckle
-    from d2ic_amd.mask2mesh import mask_to_mesh
-    from d2ic_amd.mesh_assets import compute_element_centers
-    from d2ic_amd.pixel_assets import build_pixel_assets
-    from d2ic_amd.serving import StreamServer
+    from d2ic_amd import _lib
+    from d2ic_amd.serving import DONE, StreamServer
+    from d2ic_amd.solver_global_cg import CGPlan
+    from d2ic_amd.workloads import serve_streams
 
-    rng = np.random.default_rng(seed)
-    # arrivals: one Poisson process per stream
-    arrivals = []
-    for s_ in range(n_streams):
-        t = rng.exponential(n_streams / rate)
-        while t < seconds:
-            arrivals.append((t, s_))
-            t += rng.exponential(n_streams / rate)
-    arrivals.sort()
-    nfr = [sum(1 for _, k in arrivals if k == s_) for s_ in range(n_streams)]
     t0 = time.perf_counter()
-    streams = []
-    for s_ in range(n_streams):
-        side = int(np.exp(rng.uniform(np.log(256), np.log(4096))))
-        side -= side % 2
-        H = side + 64
-        e = int(rng.integers(8, 33))
-        mask = speckle.rect_mask(H, H, side, side)
-        mesh = mask_to_mesh(mask, e)
-        A, b, amp, wl = _random_motion(rng, side, max_grad=5e-3 / max(nfr[s_], 1), max_amp=3.0)
-        fns = [speckle.affine_field(A * (f + 1), b * (f + 1) / max(nfr[s_], 1), amp * (f + 1) / max(nfr[s_], 1),
-                                    wl, H / 2.0, H / 2.0) for f in range(max(nfr[s_], 1))]
-        ref, frames = speckle.speckle_sequence_device(H, H, seed * 100 + s_, fns, noise_sigma=1.0)
-        pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=mask)
-
-        class _A:
-            element_centers_xy = compute_element_centers(mesh)
-
-        u0 = _zncc_init(mesh, _A, ref, frames[0])
-        streams.append(dict(side=side, e=e, pa=pa, ref=ref, frames=frames, u0=u0))
-    torch.cuda.synchronize()
+    streams, arrivals = serve_streams(seed=seed, n_streams=n_streams, rate=rate, seconds=seconds)
     log(f"[serve] {n_streams} streams prepared in {time.perf_counter() - t0:.1f} s; {len(arrivals)} requests; "
         f"sides {[s['side'] for s in streams]}")
 
     # served: continuous batching
-    srv = StreamServer(slots=min(64, n_streams), hbm_budget_bytes=200e9, idle_ms=2.0)
+    srv = StreamServer(slots=min(64, n_streams), hbm_budget_bytes=200e9, idle_ms=2.0, max_queue=1 << 20)
     for k, s_ in enumerate(streams):
         if not srv.add_stream(k, s_["pa"], s_["ref"], reg_strength=reg, max_iter=max_iters, tol=tol, u0=s_["u0"]):
             raise RuntimeError("stream does not fit the HBM budget")
     fidx = [0] * n_streams
     reqs = []
+    depth = []  # (time, requests arrived and not finished)
     t_start = time.perf_counter()
     i = 0
+    next_sample = 0.0
     while i < len(arrivals) or srv.busy():
         now = time.perf_counter() - t_start
         while i < len(arrivals) and arrivals[i][0] <= now:
@@ -277,16 +211,27 @@ def serving_leg(seed=11, n_streams=16, rate=200.0, seconds=1.0, max_iters=200, t
             fidx[k] += 1
             i += 1
         srv.pump()
-        if time.perf_counter() - t_start > 300:
+        if now >= next_sample:
+            depth.append((now, i - len(srv.done) - len(srv.unsolved)))
+            next_sample += 0.05
+        if now > 300 + 3 * seconds:
             raise RuntimeError("serving leg timed out")
+    bad = [r for r in reqs if r.status != DONE]
+    if bad:
+        raise RuntimeError(f"{len(bad)} served requests did not finish: {bad[0].status} {bad[0].reason}")
     t_end = max(r.t_done for r in reqs)
     lat_s = np.array([r.t_done - r.t_arrival for r in reqs])
     served_u = [r.u for r in reqs]
     srv.close()
+    # queue growth over the arrival window: depth in its last fifth vs its second fifth
+    d_arr = np.array([d for t, d in depth if t <= seconds])
+    tt = np.array([t for t, d in depth if t <= seconds])
+    q2 = float(d_arr[(tt > 0.2 * seconds) & (tt <= 0.4 * seconds)].mean()) if d_arr.size else 0.0
+    q5 = float(d_arr[tt > 0.8 * seconds].mean()) if d_arr.size else 0.0
+    done_in_window = sum(1 for r in reqs if r.t_done - t_start <= seconds)
+    bounded = q5 <= max(2.0 * q2, q2 + n_streams)
 
     # plain: one solve per launch, first come first served
-    from d2ic_amd.solver_global_cg import CGPlan
-
     plans = []
     for s_ in streams:
         pl = CGPlan(s_["pa"])
@@ -318,14 +263,22 @@ def serving_leg(seed=11, n_streams=16, rate=200.0, seconds=1.0, max_iters=200, t
     n = len(arrivals)
     res = dict(streams=n_streams, offered_rate=rate, seconds=seconds, requests=n,
                sides=[s_["side"] for s_ in streams], elements=[s_["e"] for s_ in streams],
+               mpx_per_request_mean=float(np.mean([streams[k]["pa"].pixel_coords_ref.shape[0] for _, k in arrivals]))
+               / 1e6,
                served_frames_per_s=n / (t_end - t_start), plain_frames_per_s=n / (t_end_p - t_start_p),
+               served_completed_in_window_per_s=done_in_window / seconds,
+               queue_depth_2nd_fifth=q2, queue_depth_last_fifth=q5, queue_bounded=bool(bounded),
                served_p50_ms=1e3 * float(np.percentile(lat_s, 50)), served_p99_ms=1e3 * float(np.percentile(lat_s, 99)),
                plain_p50_ms=1e3 * float(np.percentile(lat_p, 50)), plain_p99_ms=1e3 * float(np.percentile(lat_p, 99)),
+               latency_note=("steady state: percentiles are load latencies" if bounded else
+                             "queue grew through the window: the offered rate exceeds the sustained rate, so the "
+                             "percentiles measure queue growth, not load latency"),
                hbm_budget_bytes=200e9, slots=min(64, n_streams), bitwise_equal=bool(same),
+               accuracy="served results are bitwise the plain per-solve results (same accuracy)",
                init="ZNCC (win 41, search 4) + IDW on each stream's first frame, before the timed run")
     log(f"[serve] served {res['served_frames_per_s']:.1f} fr/s p50 {res['served_p50_ms']:.1f} ms "
-        f"p99 {res['served_p99_ms']:.1f} ms | plain {res['plain_frames_per_s']:.1f} fr/s "
-        f"p50 {res['plain_p50_ms']:.1f} ms p99 {res['plain_p99_ms']:.1f} ms")
+        f"p99 {res['served_p99_ms']:.1f} ms (queue {q2:.1f} -> {q5:.1f}, bounded {bounded}) | plain "
+        f"{res['plain_frames_per_s']:.1f} fr/s p50 {res['plain_p50_ms']:.1f} ms p99 {res['plain_p99_ms']:.1f} ms")
     return res
 
 
@@ -426,8 +379,9 @@ def local_refine_leg(dic, ref, frame, u0, n_pixels, sweeps=3, reps=5):
     import torch
 
     from d2ic_amd import DICMeshBased, LocalGaussNewtonSolver, MeshDICConfig
+    from d2ic_amd.workloads import PLATE
 
-    w = WORKLOAD
+    w = PLATE
     solver = LocalGaussNewtonSolver(lam=0.1, max_step=0.2, omega=0.5, interpolation=w["interp"])
     loc = DICMeshBased(dic.mesh, solver, MeshDICConfig(max_iters=sweeps, tol=w["tol"], reg_strength=w["reg"],
                                                        save_history=True))
@@ -496,25 +450,151 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(ref_np, frames_np, mesh, mask, n_frames: int):
-    """The oracle (a CPU restatement of the reference algorithm, FP32) on the host cores."""
+def _cpu_model():
+    """`lscpu` model name of the host (or the platform string)."""
+    import platform
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def rect_leg(reps=5):
+    """The 512x512 single-frame workload end to end on the GPU: ZNCC (win 41,
+    search 4) at the element centres, IDW to the nodes, then the global solve
+    (cubic, 200 iterations max, tol 1e-3, reg 0.1) and strain; median of
+    ``reps`` runs after one untimed run.  Returns the timing and the arrays the
+    CPU baseline compares against."""
+    import torch
+
+    from d2ic_amd import (DICInitMotion, DICMeshBased, GlobalCGSolver, InitMotionConfig, MeshDICConfig,
+                          TranslationZNCCSolver, mask_to_mesh_assets)
+    from d2ic_amd.workloads import RECT, rect_single
+
+    w = RECT
+    ref_np, de_np, mask = rect_single()
+    ref = torch.as_tensor(ref_np, device="cuda")
+    de = torch.as_tensor(de_np, device="cuda")
+    mesh, assets = mask_to_mesh_assets(mask, w["e"])
+    init = DICInitMotion(InitMotionConfig(win=w["win"], search=w["search"]),
+                         TranslationZNCCSolver(InitMotionConfig(win=w["win"], search=w["search"])))
+    dic = DICMeshBased(mesh, GlobalCGSolver(interpolation=w["interp"]),
+                       MeshDICConfig(max_iters=w["max_iters"], tol=w["tol"], reg_strength=w["reg"]))
+    dic.prepare(ref, assets)
+    init.prepare(ref, dic.state.assets)
+
+    def once():
+        u0 = init.run(de).u_nodal
+        dic.set_initial_guess(u0)
+        return dic.run(de)
+
+    once()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = once()
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    dic.solver.close()
+    return dict(ms=float(np.median(ms)), n_iters=int(res.diagnostics.info["n_iters"]),
+                u=res.u_nodal.cpu().numpy(), strain=res.strain.cpu().numpy(), mesh=mesh, mask=mask,
+                ref=ref_np, de=de_np)
+
+
+def cpu_baseline_and_accuracy(ref_np, frames_np, mesh, mask, gpu_frames, n_timed, rect):
+    """CPU baseline and accuracy block.
+
+    * 4 Mpx plate (the headline workload): the FP32 oracle (a CPU restatement
+      of the reference algorithm, C + OpenMP on every host core) solves frame 1
+      (warm-up, untimed) and then frames 2..1+n_timed warm-started, each timed;
+      the baseline is 1 / median frame time.  The FP64 oracle runs the same
+      chain for the accuracy block.
+    * 512x512 single frame: ZNCC + IDW + global solve in the FP32 oracle, one
+      untimed run, then the median of 5.
+    * accuracy: max |u_gpu - u_oracle| over the nodes that pixels touch, per
+      frame, against the FP32 and FP64 chains, and the strain difference."""
     from oracle import oracle as O
 
-    w = WORKLOAD
-    t0 = time.perf_counter()
+    from d2ic_amd.mesh_assets import build_node_neighbor_tables, compute_element_centers
+    from d2ic_amd.workloads import PLATE, RECT
+
+    w = PLATE
     a = O.pixel_assets(mesh.nodes_xy, mesh.elements, ref_np.shape, roi_mask=mask)
-    t_assets = time.perf_counter() - t0
-    u = np.zeros((mesh.nodes_xy.shape[0], 2), np.float32)
-    iters = []
-    t0 = time.perf_counter()
-    for k in range(n_frames):
-        u, _, n = O.cg_solve(u, ref_np, frames_np[k], a, alpha_reg=w["reg"], max_iter=w["max_iters"], tol=w["tol"],
-                             interp=w["interp"], dtype=np.float32)
-        iters.append(n)
-    dt = time.perf_counter() - t0
-    return dict(value=n_frames / dt, unit="frames/s", cores=O.num_threads(), kind="port",
-                sample=f"first {n_frames} frame(s) of the same sequence, FP32 oracle (C, OpenMP), "
-                       f"iterations {iters}, {dt:.1f} s; asset build {t_assets:.1f} s not included")
+    used = a["node_pixel_degree"] > 0
+    nb_idx, nb_deg = (np.asarray(x) for x in build_node_neighbor_tables(mesh))
+    X = np.asarray(mesh.nodes_xy, np.float32)
+    chains = {}
+    frame_s = []
+    for name, dt in (("fp32", np.float32), ("fp64", np.float64)):
+        u = np.zeros((X.shape[0], 2), dt)
+        us, its = [], []
+        for k in range(n_timed + 1):
+            t0 = time.perf_counter()
+            u, _, n = O.cg_solve(u, ref_np, frames_np[k], a, alpha_reg=w["reg"], max_iter=w["max_iters"],
+                                 tol=w["tol"], interp=w["interp"], dtype=dt)
+            if name == "fp32" and k > 0:
+                frame_s.append(time.perf_counter() - t0)
+            us.append(u.copy())
+            its.append(n)
+        chains[name] = (us, its)
+        log(f"[cpu] {name} chain: iterations {its}")
+    per_frame = []
+    for k, (ug, sg) in enumerate(gpu_frames[: n_timed + 1]):
+        u32, u64 = chains["fp32"][0][k], chains["fp64"][0][k]
+        _, _, v32 = O.strain(u32, X, nb_idx, nb_deg)
+        per_frame.append(dict(frame=k + 1, max_abs_du_vs_fp32_px=float(np.abs(ug - u32)[used].max()),
+                              max_abs_du_vs_fp64_px=float(np.abs(ug - u64)[used].max()),
+                              max_abs_du_fp32_vs_fp64_px=float(np.abs(u32 - u64)[used].max()),
+                              max_abs_dstrain_vs_fp32=float(np.abs(sg - v32)[used].max())))
+    t_med = float(np.median(frame_s))
+
+    # 512x512 single frame, end to end
+    r = RECT
+    ra = O.pixel_assets(rect["mesh"].nodes_xy, rect["mesh"].elements, rect["ref"].shape, roi_mask=rect["mask"])
+    cen = np.asarray(compute_element_centers(rect["mesh"]), np.float32)
+    rused = ra["node_pixel_degree"] > 0
+
+    def rect_once(dt):
+        uc, sc = O.zncc(cen, rect["ref"], rect["de"], r["win"], r["search"])
+        uc[~(sc >= 0.2)] = 0.0  # InitMotionConfig.score_min default (solver_translation_zncc.py:74-77)
+        u0 = O.idw(rect["mesh"].nodes_xy, cen, uc)
+        return O.cg_solve(u0, rect["ref"], rect["de"], ra, alpha_reg=r["reg"], max_iter=r["max_iters"],
+                          tol=r["tol"], interp=r["interp"], dtype=dt)
+
+    rect_once(np.float32)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ur32, _, nr32 = rect_once(np.float32)
+        ts.append(time.perf_counter() - t0)
+    ur64, _, nr64 = rect_once(np.float64)
+    rect_acc = dict(max_abs_du_vs_fp32_px=float(np.abs(rect["u"] - ur32)[rused].max()),
+                    max_abs_du_vs_fp64_px=float(np.abs(rect["u"] - ur64)[rused].max()),
+                    max_abs_du_fp32_vs_fp64_px=float(np.abs(ur32 - ur64)[rused].max()),
+                    gpu_iters=rect["n_iters"], oracle_fp32_iters=int(nr32), oracle_fp64_iters=int(nr64))
+    cores = O.num_threads()
+    model = _cpu_model()
+    baseline = dict(value=1.0 / t_med, unit="frames/s", cores=cores, kind="port",
+                    sample=(f"4 Mpx plate: FP32 oracle (C, OpenMP, {cores} threads, {model}), frame 1 untimed, "
+                            f"then median of frames 2..{n_timed + 1} warm-started ({t_med:.2f} s per frame; "
+                            f"iterations {chains['fp32'][1]}); asset build not included"),
+                    frame_s=frame_s, cpu_model=model, nproc=os.cpu_count(),
+                    rect512=dict(frames_per_s=1.0 / float(np.median(ts)), ms=1e3 * float(np.median(ts)),
+                                 sample="512x512 single frame: ZNCC + IDW + global solve, FP32 oracle, one untimed "
+                                        "run then the median of 5", gpu_ms=rect["ms"],
+                                 gpu_vs_cpu=float(np.median(ts)) * 1e3 / rect["ms"]))
+    accuracy = dict(plate_frames=per_frame, rect512=rect_acc,
+                    bars="nodal u within 1e-4 px of the FP32 oracle; strain within 1e-6; the FP64 error is stated",
+                    note="nodes touched by pixels; the GPU chain and each oracle chain warm-start from their own "
+                         "previous frame")
+    return baseline, accuracy
 
 
 def main() -> int:
@@ -522,11 +602,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-frames", type=int, default=5)
+    ap.add_argument("--cpu-frames", type=int, default=5,
+                    help="CPU baseline: frames timed after one untimed frame (also the frames of the accuracy block)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-streams leg")
-    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the served leg at the offered 200 frame pairs/s")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-local", action="store_true", help="skip the local Gauss-Newton refinement leg")
     ap.add_argument("--autotune", action="store_true",
@@ -552,12 +634,14 @@ def main() -> int:
         return 0
 
     from d2ic_amd import DICMeshBased, GlobalCGSolver, MeshDICConfig, mask_to_mesh_assets
+    from d2ic_amd.workloads import PLATE, plate_sequence
 
     torch.cuda.init()
-    w = WORKLOAD
+    w = PLATE
     K, Wm = int(args.steps), int(args.warmup)
+    n_acc = max(0, min(K - 1, args.cpu_frames)) if not args.no_cpu_baseline else 0
     log(f"[bench] rendering {K} frames of {w['H']}x{w['W']} speckle on the GPU")
-    ref, frames, mask = build_workload(max(K, 1))
+    ref, frames, mask = plate_sequence(max(K, 1))
     mesh, assets = mask_to_mesh_assets(mask, w["e"])
     cfg = MeshDICConfig(max_iters=w["max_iters"], tol=w["tol"], reg_strength=w["reg"], save_history=False)
     solver = GlobalCGSolver(interpolation=w["interp"])
@@ -584,7 +668,7 @@ def main() -> int:
         torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
-    kt, kms, passes, iters = [], [], [], []
+    kt, kms, passes, iters, kept = [], [], [], [], []
     u_prev = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -596,6 +680,8 @@ def main() -> int:
         res = dic.run(frames[k])
         ev1.record(stream)
         u_prev = res.u_nodal
+        if k <= n_acc:
+            kept.append((res.u_nodal.clone(), res.strain.clone()))
         kt.append((ev0, ev1))
         passes.append(solver.last_pass_count)
         kms.append(solver.plan.kernel_ms())
@@ -631,19 +717,23 @@ def main() -> int:
         log("[bench] served leg: 16 streams, offered 200 frame pairs/s")
         serving = {"offered_200_per_s": serving_leg(rate=200.0, seconds=args.serve_seconds)}
         log("[bench] served leg: 16 streams, offered 50 frame pairs/s")
-        serving["offered_50_per_s"] = serving_leg(rate=50.0, seconds=2.0 * args.serve_seconds)
+        serving["offered_50_per_s"] = serving_leg(rate=50.0, seconds=min(4.0, args.serve_seconds))
 
     diff = None
     if not args.no_diff:
         log("[bench] differentiation leg: 1024x1024")
         diff = differentiation_leg()
 
-    cpu = None
-    if not args.no_cpu_baseline and args.cpu_frames > 0:
-        log(f"[bench] CPU baseline: oracle on {args.cpu_frames} frame(s)")
+    cpu = accuracy = None
+    if n_acc > 0:
+        log("[bench] 512x512 single frame on the GPU (ZNCC + IDW + solve)")
+        rect = rect_leg()
+        log(f"[bench] CPU baseline and accuracy: oracle FP32/FP64 chains over {n_acc + 1} frames, 512x512 frame")
         ref_np = ref.cpu().numpy()
-        fr_np = [f.cpu().numpy() for f in frames[: args.cpu_frames]]
-        cpu = cpu_baseline(ref_np, fr_np, mesh, mask, args.cpu_frames)
+        fr_np = [f.cpu().numpy() for f in frames[: n_acc + 1]]
+        gpu_frames = [(u.cpu().numpy(), e.cpu().numpy()) for u, e in kept]
+        cpu, accuracy = cpu_baseline_and_accuracy(ref_np, fr_np, mesh, mask, gpu_frames, n_acc, rect)
+        log(f"[bench] accuracy: {accuracy}")
 
     line = {
         "metric": "frames/s at 4 Mpx Q4-16px; p50/p99 latency under load vs plain; HBM GB/s",
@@ -681,6 +771,7 @@ def main() -> int:
             "mean_solve_ms": float(np.mean(solve_ms)),
         },
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
         "batch": batch,
         "serving": serving,
         "differentiation": diff,
