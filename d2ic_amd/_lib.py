@@ -43,6 +43,9 @@ _SIGS = {
     "d2ic_cg_solve": [_vp, _vp, _i32, _i32, _vp, _f32, _i32, _f32, _vp, _vp, _vp, _vp],
     "d2ic_cg_solve_batch": [_i32, _pp, _pp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), _pp,
                             ctypes.POINTER(_f32), ctypes.POINTER(_i32), ctypes.POINTER(_f32), _pp, _pp, _pp, _vp],
+    "d2ic_cg_solve_batch_noalloc": [_i32, _pp, _pp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), _pp,
+                                    ctypes.POINTER(_f32), ctypes.POINTER(_i32), ctypes.POINTER(_f32), _pp, _pp, _pp,
+                                    _vp],
     "d2ic_cg_plan_set_grid": [_vp, _i32],
     "d2ic_cg_kernel_ms": [_vp, ctypes.POINTER(_f32)],
     "d2ic_serve_create": [_i32, ctypes.c_double, ctypes.POINTER(_vp)],

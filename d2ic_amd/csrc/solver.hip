@@ -2069,18 +2069,21 @@ int d2ic_cg_set_reference(d2ic_cg_plan* plan, const float* ref_image, int32_t H,
     return D2IC_OK;
 }
 
-int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const float* const* def_images,
-                        const int32_t* H, const int32_t* W, float* const* u, const float* alpha_reg,
-                        const int32_t* max_iter, const float* tol, float* const* hist, int32_t* const* n_iters,
-                        int32_t* const* pass_count, void* stream) {
-    hipStream_t st = (hipStream_t)stream;
-    const int P = n_problems;
-    D2IC_CHECK_ARG(P >= 1 && P <= kMaxProblems, "n_problems must be in [1, %d]", kMaxProblems);
+}  // extern "C"
+
+namespace {
+
+// Validate a batch and build its descriptors (shared by the allocating and
+// the allocation-free entry points).
+int batch_descs(int32_t P, d2ic_cg_plan* const* plans, const float* const* def_images, const int32_t* H,
+                const int32_t* W, float* const* u, const float* alpha_reg, const int32_t* max_iter, const float* tol,
+                float* const* hist, int32_t* const* n_iters, int32_t* const* pass_count, ProbDesc* hd, int* max_it,
+                int64_t* total_work) {
     D2IC_CHECK_ARG(plans && def_images && H && W && u && alpha_reg && max_iter && tol && n_iters,
                    "batch arrays must not be NULL");
-    std::vector<ProbDesc> hd(P);
-    int max_it = 0;
-    int64_t total_items = 0, total_work = 0;
+    int64_t total_items = 0;
+    *max_it = 0;
+    *total_work = 0;
     for (int p = 0; p < P; ++p) {
         d2ic_cg_plan* pl = plans[p];
         D2IC_CHECK_ARG(pl != nullptr, "plan %d is NULL", p);
@@ -2095,39 +2098,20 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
         D2IC_CHECK_ARG(max_iter[p] >= 0 && max_iter[p] <= (1 << 20), "max_iter[%d] out of range", p);
         hd[p] = make_desc(pl, def_images[p], H[p], W[p], u[p], alpha_reg[p], max_iter[p], tol[p],
                           hist ? hist[p] : nullptr, n_iters[p], pass_count ? pass_count[p] : nullptr);
-        max_it = std::max(max_it, max_iter[p]);
+        *max_it = std::max(*max_it, max_iter[p]);
         total_items += pl->ntiles + pl->nchunks;
-        total_work += pl->ntiles * kChunks + pl->nchunks;
+        *total_work += pl->ntiles * kChunks + pl->nchunks;
     }
     D2IC_CHECK_ARG(total_items < (int64_t)INT32_MAX, "batch too large");
+    return D2IC_OK;
+}
 
-    const size_t off_S = ((sizeof(ProbDesc) * P + 255) / 256) * 256;
-    const size_t off_W = off_S + ((sizeof(ProbState) * P + 255) / 256) * 256;
-    const size_t bytes = off_W + ((sizeof(Workspace) + 255) / 256) * 256;
-    Tmp mem;
-    D2IC_HIP(mem.alloc(bytes, st));
-    char* base = mem.as<char>();
-    ProbDesc* d_desc = reinterpret_cast<ProbDesc*>(base);
-    ProbState* d_S = reinterpret_cast<ProbState*>(base + off_S);
-    Workspace* d_ws = reinterpret_cast<Workspace*>(base + off_W);
-
-    Staging& sg = g_stage;
-    if (sg.pending) {
-        D2IC_HIP(hipEventSynchronize(sg.ev));
-        sg.pending = false;
-    }
-    if (sg.cap < sizeof(ProbDesc) * P) {
-        if (sg.host) (void)hipHostFree(sg.host);
-        sg.host = nullptr;
-        sg.cap = 0;
-        D2IC_HIP(hipHostMalloc(&sg.host, sizeof(ProbDesc) * kMaxProblems, hipHostMallocDefault));
-        sg.cap = sizeof(ProbDesc) * kMaxProblems;
-    }
-    if (!sg.ev) D2IC_HIP(hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming));
-    memcpy(sg.host, hd.data(), sizeof(ProbDesc) * P);
-    D2IC_HIP(hipMemcpyAsync(d_desc, sg.host, sizeof(ProbDesc) * P, hipMemcpyHostToDevice, st));
-    D2IC_HIP(hipEventRecord(sg.ev, st));
-    sg.pending = true;
+// Launch sequence of a solve whose descriptors are already in d_desc: clear
+// the workspace, per-problem start state, the persistent kernel, finalize.
+// Allocates nothing; `timed` records the plan's start/end events around the
+// persistent launch (creating them on first use).
+int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, ProbDesc* d_desc, ProbState* d_S,
+                 Workspace* d_ws, int max_it, int64_t total_work, hipStream_t st, bool timed) {
     static_assert(sizeof(Workspace) % 4 == 0, "workspace is cleared in 32-bit words");
     k_zero_words<<<64, 256, 0, st>>>(reinterpret_cast<uint32_t*>(d_ws), (int64_t)(sizeof(Workspace) / 4));
     D2IC_LAUNCH_CHECK();
@@ -2156,9 +2140,11 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     ServeHost* no_sh = nullptr;
     void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps, (void*)&no_sh};
     d2ic_cg_plan* lead = plans[0];
-    if (!lead->ev_start) D2IC_HIP(hipEventCreate(&lead->ev_start));
-    if (!lead->ev_end) D2IC_HIP(hipEventCreate(&lead->ev_end));
-    D2IC_HIP(hipEventRecord(lead->ev_start, st));
+    if (timed) {
+        if (!lead->ev_start) D2IC_HIP(hipEventCreate(&lead->ev_start));
+        if (!lead->ev_end) D2IC_HIP(hipEventCreate(&lead->ev_end));
+        D2IC_HIP(hipEventRecord(lead->ev_start, st));
+    }
     // D2IC_PLAIN_LAUNCH=1 launches the same grid with an ordinary launch (for
     // tools that cannot follow cooperative launches).  The grid never exceeds
     // the co-resident capacity; if blocks were not all resident the bounded
@@ -2176,11 +2162,143 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
     } else {
         D2IC_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kBlock), args, 0, st));
     }
-    D2IC_HIP(hipEventRecord(lead->ev_end, st));
-    lead->timed = true;
+    if (timed) {
+        D2IC_HIP(hipEventRecord(lead->ev_end, st));
+        lead->timed = true;
+    } else {
+        lead->timed = false;  // d2ic_cg_kernel_ms reports no timed launch rather than an older one
+    }
     k_finalize<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_ws);
     D2IC_LAUNCH_CHECK();
     return D2IC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free solves (d2ic_cg_solve_batch_noalloc): the per-launch
+// descriptors, solver states and workspace live in one file-scope device
+// array per GPU instead of a hipMallocAsync block.  Its device address is
+// looked up once per device with hipGetSymbolAddress; host code uses only that
+// address, never the array's name (which in host code is a stand-in, not a
+// device pointer).  Every call on a GPU shares its array, so calls that
+// overlap on one GPU (concurrent streams or threads) race.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocProblems = 8;
+constexpr size_t kScratchOffS = ((sizeof(ProbDesc) * kNoAllocProblems + 255) / 256) * 256;
+constexpr size_t kScratchOffW = kScratchOffS + ((sizeof(ProbState) * kNoAllocProblems + 255) / 256) * 256;
+constexpr size_t kScratchBytes = kScratchOffW + ((sizeof(Workspace) + 255) / 256) * 256;  // ~0.47 MB
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct DescPack {
+    ProbDesc d[kNoAllocProblems];
+};
+static_assert(sizeof(DescPack) <= 4000, "descriptors travel as one kernel argument");
+
+// Copy the descriptors, passed by value as a kernel argument, into the scratch
+// array (no host staging buffer and no host-to-device copy).
+__global__ void k_store_descs(DescPack pack, int P, ProbDesc* __restrict__ out) {
+    constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
+    static_assert(sizeof(ProbDesc) % 8 == 0, "descriptors are copied in 8-byte words");
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&pack.d[0]);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(out);
+    for (int i = threadIdx.x; i < P * kWords; i += blockDim.x) dst[i] = src[i];
+}
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // per device: device address of `scratch`
+
+int scratch_address(unsigned char** out) {
+    int dev = 0;
+    D2IC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return D2IC_ERR_DEVICE;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return D2IC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const float* const* def_images,
+                        const int32_t* H, const int32_t* W, float* const* u, const float* alpha_reg,
+                        const int32_t* max_iter, const float* tol, float* const* hist, int32_t* const* n_iters,
+                        int32_t* const* pass_count, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int P = n_problems;
+    D2IC_CHECK_ARG(P >= 1 && P <= kMaxProblems, "n_problems must be in [1, %d]", kMaxProblems);
+    std::vector<ProbDesc> hd(P);
+    int max_it = 0;
+    int64_t total_work = 0;
+    int rc = batch_descs(P, plans, def_images, H, W, u, alpha_reg, max_iter, tol, hist, n_iters, pass_count, hd.data(),
+                         &max_it, &total_work);
+    if (rc != D2IC_OK) return rc;
+
+    const size_t off_S = ((sizeof(ProbDesc) * P + 255) / 256) * 256;
+    const size_t off_W = off_S + ((sizeof(ProbState) * P + 255) / 256) * 256;
+    const size_t bytes = off_W + ((sizeof(Workspace) + 255) / 256) * 256;
+    Tmp mem;
+    D2IC_HIP(mem.alloc(bytes, st));
+    char* base = mem.as<char>();
+    ProbDesc* d_desc = reinterpret_cast<ProbDesc*>(base);
+    ProbState* d_S = reinterpret_cast<ProbState*>(base + off_S);
+    Workspace* d_ws = reinterpret_cast<Workspace*>(base + off_W);
+
+    Staging& sg = g_stage;
+    if (sg.pending) {
+        D2IC_HIP(hipEventSynchronize(sg.ev));
+        sg.pending = false;
+    }
+    if (sg.cap < sizeof(ProbDesc) * P) {
+        if (sg.host) (void)hipHostFree(sg.host);
+        sg.host = nullptr;
+        sg.cap = 0;
+        D2IC_HIP(hipHostMalloc(&sg.host, sizeof(ProbDesc) * kMaxProblems, hipHostMallocDefault));
+        sg.cap = sizeof(ProbDesc) * kMaxProblems;
+    }
+    if (!sg.ev) D2IC_HIP(hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming));
+    memcpy(sg.host, hd.data(), sizeof(ProbDesc) * P);
+    D2IC_HIP(hipMemcpyAsync(d_desc, sg.host, sizeof(ProbDesc) * P, hipMemcpyHostToDevice, st));
+    D2IC_HIP(hipEventRecord(sg.ev, st));
+    sg.pending = true;
+    return launch_solve(P, plans, max_iter, d_desc, d_S, d_ws, max_it, total_work, st, true);
+}
+
+int d2ic_cg_solve_batch_noalloc(int32_t n_problems, d2ic_cg_plan* const* plans, const float* const* def_images,
+                                const int32_t* H, const int32_t* W, float* const* u, const float* alpha_reg,
+                                const int32_t* max_iter, const float* tol, float* const* hist,
+                                int32_t* const* n_iters, int32_t* const* pass_count, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int P = n_problems;
+    D2IC_CHECK_ARG(P >= 1 && P <= kNoAllocProblems,
+                   "n_problems must be in [1, %d] for the allocation-free solve (its device scratch array holds "
+                   "%d problems); use d2ic_cg_solve_batch for more",
+                   kNoAllocProblems, kNoAllocProblems);
+    DescPack pack;
+    memset(&pack, 0, sizeof(pack));
+    int max_it = 0;
+    int64_t total_work = 0;
+    int rc = batch_descs(P, plans, def_images, H, W, u, alpha_reg, max_iter, tol, hist, n_iters, pass_count, pack.d,
+                         &max_it, &total_work);
+    if (rc != D2IC_OK) return rc;
+    unsigned char* base = nullptr;
+    rc = scratch_address(&base);
+    if (rc != D2IC_OK) return rc;
+    ProbDesc* d_desc = reinterpret_cast<ProbDesc*>(base);
+    ProbState* d_S = reinterpret_cast<ProbState*>(base + kScratchOffS);
+    Workspace* d_ws = reinterpret_cast<Workspace*>(base + kScratchOffW);
+    k_store_descs<<<1, 256, 0, st>>>(pack, P, d_desc);
+    D2IC_LAUNCH_CHECK();
+    return launch_solve(P, plans, max_iter, d_desc, d_S, d_ws, max_it, total_work, st, false);
 }
 
 int d2ic_cg_solve(d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W, float* u, float alpha_reg,

@@ -113,6 +113,22 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
                         const int32_t* max_iter, const float* tol, float* const* hist, int32_t* const* n_iters,
                         int32_t* const* pass_count, void* stream);
 
+/* d2ic_cg_solve_batch without any allocation (no reference counterpart): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results (bitwise) as d2ic_cg_solve_batch, for at most 8
+ * problems: the per-launch descriptors, solver states and workspace live in
+ * one static device array of ~0.47 MB per GPU (held until the process ends)
+ * instead of a hipMallocAsync block, and descriptors travel as a kernel
+ * argument instead of through pinned staging memory.  More than 8 problems
+ * is rejected with D2IC_ERR_INVALID.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams or threads) race.  It
+ * records no timing events (d2ic_cg_kernel_ms then reports no timed launch). */
+int d2ic_cg_solve_batch_noalloc(int32_t n_problems, d2ic_cg_plan* const* plans, const float* const* def_images,
+                                const int32_t* H, const int32_t* W, float* const* u, const float* alpha_reg,
+                                const int32_t* max_iter, const float* tol, float* const* hist,
+                                int32_t* const* n_iters, int32_t* const* pass_count, void* stream);
+
 /* Grid size (blocks, rounded up to a multiple of 8 and capped by what can be
  * co-resident) of the persistent kernel for single-problem solves with this
  * plan; 0 restores the work-based default.  Results do not depend on it
