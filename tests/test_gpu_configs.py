@@ -11,6 +11,8 @@ FP64 run for its x64 run).
   first 8 iterates against the FP64 oracle.
 * The 16-stream serving mix: every served request bitwise equal to the plain
   per-solve path.
+* 1024x1024 differentiation: the adjoint's image and reg_strength gradients
+  and a noise-probe tangent against the FP64 oracle's finite differences.
 
 Bars (north star): integer ZNCC shifts bit-exact; nodal u within 1e-4 px of
 the FP32 oracle; Green-Lagrange strain within 1e-6; the FP64 error is printed
@@ -263,3 +265,112 @@ def test_config_serve16_mix(gpu):
         fidx[k] += 1
     for pl in plans:
         pl.close()
+
+
+@pytest.mark.timeout(900)
+def test_config_diff1024_adjoint(gpu):
+    """1024x1024 pair, Q4-16 on an 800x800 ROI, the plate field at frame 10.
+    Loss = |u - u_true|^2 over the moving nodes + mean E_xx over the nodes of a
+    200x200 window.  At the FP64 minimiser u* (oracle NLCG + Newton), the HIP
+    adjoint's dL/dg and dL/df at 32 random pixels each and dL/d reg_strength
+    are checked against the implicit-function derivative built from the FP64
+    oracle alone: -lambda . dG/dtheta, with dG/dtheta from central differences
+    of the oracle gradient (h = 1e-2 grey per pixel, 1e-4 for reg_strength) and
+    lambda = H^-1 dL/du from CG on central-difference Hessian-vector products
+    of the same gradient.  One noise probe's forward tangent is checked the
+    same way.  Bar: 2e-3 of max |reference derivative| (the reference's JAX
+    gradients cannot run here: parity against them is unpinned)."""
+    from d2ic_amd import speckle
+    from d2ic_amd.adjoint import Linearization, green_lagrange_torch
+    from d2ic_amd.mask2mesh import mask_to_mesh
+    from d2ic_amd.mesh_assets import build_node_neighbor_tables
+    from d2ic_amd.pixel_assets import build_pixel_assets
+    from d2ic_amd.solver_global_cg import CGPlan
+    from tests._fd import cg_fd, oracle_grad, tight_solution
+
+    print("[case] 1024^2 differentiation: adjoint vs oracle finite differences", flush=True)
+    t = gpu
+    H = W = 1024
+    alpha = 0.1
+    mask = speckle.rect_mask(H, W, 800, 800)
+    mesh = mask_to_mesh(mask, 16)
+    fn = speckle.affine_sinusoid(10, W, H)
+    ref_d, frames = speckle.speckle_sequence_device(H, W, 3, [fn], noise_sigma=1.0)
+    g_d = frames[0]
+    ref, de = ref_d.cpu().numpy(), g_d.cpu().numpy()
+    pa = build_pixel_assets(mesh, ref_d, 1.0, roi_mask=mask)
+    plan = CGPlan(pa)
+    plan.set_reference(ref_d, "cubic")
+    oa = O.pixel_assets(mesh.nodes_xy, mesh.elements, (H, W), roi_mask=mask)
+    act = plan.node_active.cpu().numpy()
+    X = np.asarray(mesh.nodes_xy, np.float64)
+    ux, uy = fn(X[:, 0], X[:, 1])
+    u_true = np.stack([ux, uy], 1) * act[:, None]
+    t0 = time.perf_counter()
+    u_star = tight_solution(ref, de, oa, u_true, alpha, act[:, None], nlcg_iters=100)
+    grad = oracle_grad(ref, de, oa, alpha, act[:, None])
+    g_res = float(np.abs(grad(u_star)).max())
+    print(f"u* in {time.perf_counter() - t0:.1f} s, max|grad J(u*)| {g_res:.2e}, |u* - u_true| max "
+          f"{np.abs(u_star - u_true)[act].max():.3f} px", flush=True)
+    assert g_res < 1e-9
+
+    nb_idx, nb_deg = (np.asarray(x) for x in build_node_neighbor_tables(mesh))
+    win = act & (np.abs(X[:, 0] - W / 2) <= 100) & (np.abs(X[:, 1] - H / 2) <= 100)
+    ut = t.tensor(u_star, dtype=t.float64, device="cuda", requires_grad=True)
+    E = green_lagrange_torch(ut, X, nb_idx, nb_deg)
+    act_t, win_t = t.tensor(act, device="cuda"), t.tensor(win, device="cuda")
+    L = ((ut - t.tensor(u_true, device="cuda"))[act_t] ** 2).sum() + E[win_t, 0, 0].mean()
+    L.backward()
+    dLdu = ut.grad.cpu().numpy() * act[:, None]
+
+    # HIP adjoint
+    lin = Linearization(plan, g_d, u_star.astype(np.float32))
+    lam = lin.solve(t.tensor(dLdu[None], dtype=t.float32, device="cuda"), alpha, rtol=1e-7)[0]
+    gf, gg = (x.cpu().numpy() for x in lin.image_grads(lam))
+    galpha = float(-(lam.double() * lin.spring(t.tensor(u_star[None], dtype=t.float32, device="cuda"))[0].double())
+                   .sum())
+
+    # the oracle's implicit-function derivative
+    t0 = time.perf_counter()
+    lam_o = cg_fd(grad, u_star, dLdu, rtol=1e-10, max_iter=2000)
+    print(f"oracle lambda in {time.perf_counter() - t0:.1f} s; |lambda_gpu - lambda_oracle| / |lambda_oracle| "
+          f"{np.abs(lam.cpu().numpy() - lam_o).max() / np.abs(lam_o).max():.2e}", flush=True)
+    rng = np.random.default_rng(32)
+    rows, cols = np.nonzero(mask[140:884, 140:884])
+    pick = rng.choice(len(rows), size=32, replace=False)
+    h = 1e-2
+    for name, got in (("g", gg), ("f", gf)):
+        ref_v, adj = [], []
+        for k in pick:
+            r, c = rows[k] + 140, cols[k] + 140
+            Gs = []
+            for sgn in (1.0, -1.0):
+                rr, dd = ref.copy(), de.copy()
+                (dd if name == "g" else rr)[r, c] += np.float32(sgn * h)
+                Gs.append(oracle_grad(rr, dd, oa, alpha, act[:, None])(u_star))
+            dG = (Gs[0] - Gs[1]) / (2 * h)
+            ref_v.append(-float((lam_o * dG).sum()))
+            adj.append(float(got[r, c]))
+        ref_v, adj = np.array(ref_v), np.array(adj)
+        err = np.abs(ref_v - adj).max() / np.abs(ref_v).max()
+        print(f"dL/d{name} at 32 pixels: max |reference| {np.abs(ref_v).max():.3e}, rel err {err:.2e}", flush=True)
+        assert err < 2e-3
+    ha = 1e-4
+    dGa = (oracle_grad(ref, de, oa, alpha + ha, act[:, None])(u_star)
+           - oracle_grad(ref, de, oa, alpha - ha, act[:, None])(u_star)) / (2 * ha)
+    ra = -float((lam_o * dGa).sum())
+    print(f"dL/dalpha: reference {ra:.6e} adjoint {galpha:.6e}", flush=True)
+    assert abs(ra - galpha) < 2e-3 * abs(ra)
+
+    # one forward-tangent noise probe
+    dg = np.random.default_rng(5).normal(size=(H, W)).astype(np.float32)
+    du = lin.tangent(t.tensor(dg[None], device="cuda"), alpha, "g", rtol=1e-7)[0].cpu().numpy()
+    eps = 1e-2
+    rhs = -(oracle_grad(ref, de + eps * dg, oa, alpha, act[:, None])(u_star)
+            - oracle_grad(ref, de - eps * dg, oa, alpha, act[:, None])(u_star)) / (2 * eps)
+    du_o = cg_fd(grad, u_star, rhs, rtol=1e-10, max_iter=2000)
+    err = np.abs(du - du_o)[act].max() / np.abs(du_o[act]).max()
+    print(f"noise-probe tangent: rel err {err:.2e}", flush=True)
+    assert err < 2e-3
+    assert np.all(du[~act] == 0.0)
+    plan.close()
