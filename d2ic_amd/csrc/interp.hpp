@@ -102,17 +102,24 @@ __device__ __forceinline__ int clamp_row(int interp, int y, int H, int W) {
 __device__ __forceinline__ void sample_box(int interp, const __attribute__((address_space(3))) float* box, int bw, int lx,
                                            float fx, int ly, float fy, float& v, float& gx, float& gy) {
     if (interp == 0) {
+        // all 16 taps are requested before the weights are formed, so the LDS
+        // latency is paid once per sample rather than once per tap row
+        float I4[4][4];
+        const __attribute__((address_space(3))) float* base = box + (ly - 1) * bw + (lx - 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) I4[i][j] = base[i * bw + j];
         float wx[4], dwx[4], wy[4], dwy[4];
         keys_weights(fx, wx, dwx);
         keys_weights(fy, wy, dwy);
         float acc = 0.f, accx = 0.f, accy = 0.f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const __attribute__((address_space(3))) float* row = box + (ly - 1 + i) * bw + (lx - 1);
             float rv = 0.f, rd = 0.f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float I = row[j];
+                const float I = I4[i][j];
                 rv += wx[j] * I;
                 rd += dwx[j] * I;
             }

@@ -363,10 +363,11 @@ __device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
 constexpr int kBoxRegs = kBoxMax / kWave;
 
 // Loads of the staged box into registers (issued one tile ahead of use).
+template <int kInterp>
 __device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const TileBox& g, int lane,
                                          float (&v)[kBoxRegs]) {
     if (!g.staged) return;
-    const int n = g.bw * g.bh, interp = D->interp, H = D->H, W = D->W;
+    const int n = g.bw * g.bh, interp = kInterp >= 0 ? kInterp : D->interp, H = D->H, W = D->W;
     const D2IC_GLOBAL float* img = gld(D->img);
     const float inv_bw = 1.0f / (float)g.bw;
     // the bilinear sampler clips by the transposed shape: its taps are the
@@ -421,10 +422,12 @@ __device__ __forceinline__ void box_store(const TileBox& g, int lane, const floa
 // accumulated per lane into r^2 (FP64) and the 8 gradient partials
 // acc[2a + {0,1}] += r * (dg/dx, dg/dy) * N_a.  Samples inside the staged box
 // read LDS; others read global memory (same values).  Dead lanes add zeros.
+template <int kInterp>
 __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, const TilePix& px,
                                             const float2 (&uc)[4], const TileBox& g, bool live,
                                             const __attribute__((address_space(3))) float* box, float (&acc)[8],
                                             double& r2) {
+    const int interp = kInterp >= 0 ? kInterp : D->interp;
     const float f = px.f;
     const float4 N = px.N;
     const float ux = N.x * uc[0].x + N.y * uc[1].x + N.z * uc[2].x + N.w * uc[3].x;
@@ -437,9 +440,9 @@ __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, cons
     const bool in_box = g.staged && lx >= 1 && ly >= 1 && lx + 2 < g.bw && ly + 2 < g.bh;
     float gv, gx, gy;
     if (in_box)
-        sample_box(D->interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
+        sample_box(interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
     else
-        sample_at(D->interp, D->img, D->H, D->W, x0, fx, y0, fy, gv, gx, gy);
+        sample_at(interp, D->img, D->H, D->W, x0, fx, y0, fy, gv, gx, gy);
     const float r = live ? gv - f : 0.f;
     r2 += (double)r * (double)r;
     const float qx = r * gx, qy = r * gy;
@@ -1055,8 +1058,9 @@ template <bool kServe>
 using DescTable = typename std::conditional<kServe, ProbDesc*, const ProbDesc* __restrict__>::type;
 
 // kSingle: one problem (P == 1), so every problem lookup folds to 0 and the
-// tile pipeline carries no per-problem state.
-template <bool kServe, bool kSingle = false>
+// tile pipeline carries no per-problem state.  kInterp: 0 cubic, 1 linear for
+// every problem of the launch (the sampler is compiled in), -1 read per problem.
+template <bool kServe, bool kSingle = false, int kInterp = -1>
 __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdesc, int P, ProbState* __restrict__ S,
                                                           Workspace* __restrict__ ws, int max_steps, ServeHost* sh) {
     static_assert(!(kServe && kSingle), "a served session has many slots");
@@ -1180,7 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 tile_uc(tv_cur, s_flags[pa], s_alpha[pa], s_beta[pa], uc);
                 TileBox b_cur = tile_box(tv_cur, uc);
                 float R[kBoxRegs];
-                box_load(descs + pa, b_cur, lane, R);
+                box_load<kInterp>(descs + pa, b_cur, lane, R);
                 TilePix px = tile_pix(descs + pa, t0 - s_toff[pa], 0, lane);
                 int pb = pa, pc = pa;
                 float tv_n1 = 0.f;
@@ -1205,7 +1209,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     if (it + 1 < t1) {
                         tile_uc(tv_n1, s_flags[pb], s_alpha[pb], s_beta[pb], ucn);
                         b_nxt = tile_box(tv_n1, ucn);
-                        box_load(descs + pb, b_nxt, lane, R);
+                        box_load<kInterp>(descs + pb, b_nxt, lane, R);
                     }
                     float tv_n2 = 0.f;
                     int id_n3 = 0, pd = pc;
@@ -1225,7 +1229,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                             pn = tile_pix(Da, ta, c + 1, lane);
                         else if (it + 1 < t1)
                             pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
-                        chunk_accum(Da, px, uc, b_cur,
+                        chunk_accum<kInterp>(Da, px, uc, b_cur,
                                     c * kWave + lane < b_cur.cnt, my_box, acc, r2);
                         px = pn;
                     }
@@ -1729,9 +1733,14 @@ thread_local Staging g_stage;
 // Co-resident capacity of a persistent-kernel instance (cached per device).
 int occupancy_blocks(const void* kernel, int* out) {
     static std::mutex mu;
-    static int cached_dev[3] = {-1, -1, -1}, cached[3] = {0, 0, 0};
-    const int slot = kernel == (const void*)k_cg_persistent<true> ? 1
-                     : (kernel == (const void*)k_cg_persistent<false, true> ? 2 : 0);
+    constexpr int kKernels = 5;
+    static const void* const kern_of[kKernels] = {
+        (const void*)k_cg_persistent<false>, (const void*)k_cg_persistent<true>,
+        (const void*)k_cg_persistent<false, true, 0>, (const void*)k_cg_persistent<false, true, 1>,
+        (const void*)k_cg_persistent<false, false, 0>};
+    static int cached_dev[kKernels] = {-1, -1, -1, -1, -1}, cached[kKernels] = {0, 0, 0, 0, 0};
+    int slot = 0;
+    while (slot + 1 < kKernels && kern_of[slot] != kernel) ++slot;
     int dev = 0;
     D2IC_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(mu);
@@ -2124,8 +2133,17 @@ int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, Pro
     k_init<<<gi, 256, 0, st>>>(d_desc, P, d_S, d_ws);
     D2IC_LAUNCH_CHECK();
 
-    // a single problem runs the instance without problem lookups
-    const void* kern = P == 1 ? (const void*)k_cg_persistent<false, true> : (const void*)k_cg_persistent<false>;
+    // a single problem runs the instance without problem lookups; a launch whose
+    // problems all sample with one interpolation runs an instance with that
+    // sampler compiled in
+    int interp_all = plans[0]->interp;
+    for (int p = 1; p < P; ++p)
+        if (plans[p]->interp != interp_all) interp_all = -1;
+    const void* kern = (const void*)k_cg_persistent<false>;
+    if (P == 1)
+        kern = interp_all == 0 ? (const void*)k_cg_persistent<false, true, 0> : (const void*)k_cg_persistent<false, true, 1>;
+    else if (interp_all == 0)
+        kern = (const void*)k_cg_persistent<false, false, 0>;
     int coop_max = 0;
     int rc = occupancy_blocks(kern, &coop_max);
     if (rc != D2IC_OK) return rc;
@@ -2154,8 +2172,12 @@ int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, Pro
         return e && e[0] == '1';
     }();
     if (plain_launch) {
-        if (P == 1)
-            k_cg_persistent<false, true><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        if (kern == (const void*)k_cg_persistent<false, true, 0>)
+            k_cg_persistent<false, true, 0><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        else if (kern == (const void*)k_cg_persistent<false, true, 1>)
+            k_cg_persistent<false, true, 1><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        else if (kern == (const void*)k_cg_persistent<false, false, 0>)
+            k_cg_persistent<false, false, 0><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
         else
             k_cg_persistent<false><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
         D2IC_LAUNCH_CHECK();
