@@ -44,6 +44,7 @@ struct d2ic_cg_plan {
     float* TP = nullptr;
     float* TV = nullptr;
     double* NCP = nullptr;
+    int2* item_tr = nullptr;  // per node item: first and last tile touching its nodes
     int ref_H = 0, ref_W = 0, interp = 0;
     bool has_ref = false;
     size_t bytes = 0;

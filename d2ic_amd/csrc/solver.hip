@@ -13,23 +13,28 @@
 //     gradient is assembled deterministically (no float atomics);
 //   * the spring stencil with the combined weight w_ij * (c_i + c_j) / 2.
 //
-// Each iteration is two grid-wide steps separated by a flag barrier (every
+// Every step is one pass, and steps are separated by a flag barrier (every
 // block publishes its arrival; the first wave of block 0 waits for all of them
 // and writes every block's release word).  After each barrier every block
 // reduces the finished step from the same memory in the same order and runs
-// the same state machine, so the next step's plan is identical everywhere:
-//   A-step: every tile evaluates the residual at the candidate
-//           u + alpha * d (d formed on the fly from g, d_prev and beta) and
-//           writes its partial gradient (8 floats) and its sum of r^2; node
-//           items materialise d, and accumulate g.d and the spring energy at
-//           the candidate.
-//   B-step: the Armijo decision has been taken; if the trial is accepted,
-//           node items assemble the gradient at the new point through the
-//           gather table and reduce |g|^2 and g.(g - g_prev) for the next
-//           beta and the stop test.
-// The accepted trial's A-step is also the next iteration's value+gradient
-// evaluation, so one pixel pass per iteration is enough when alpha = 1 is
-// accepted (the reference's usual case), without changing the iterates.
+// the same state machine, so the next step's plan is identical everywhere.
+// A step of a problem evaluates one Armijo trial u + alpha d:
+//   tiles:      every tile evaluates the residual at the trial point (d formed
+//               on the fly from g, d_prev and beta), writes its partial
+//               gradient (8 floats) and adds its sum of r^2;
+//   node items: once the blocks holding the tiles of their nodes have
+//               published those partials (a wait on those blocks only, not a
+//               grid barrier), node items form d, the trial point, the
+//               gradient there (gather through the node CSR plus the spring
+//               term) and reduce |g|^2, g.(g - g_prev), g.d and the spring
+//               energy at the trial point.
+// After the barrier the Armijo test uses r^2 and the spring energy; if the
+// trial is accepted its node-item results are the next iterate, gradient and
+// direction (the next slope g.d_new = -|g|^2 + beta g.d follows from the sums,
+// so the restart rule is applied before the next pass), otherwise they are
+// discarded (they were written to the other buffers) and the next step tries
+// alpha / 2.  One barrier per trial, and one trial per iteration when
+// alpha = 1 is accepted (the reference's usual case).
 //
 // Reductions are deterministic and independent of the batch composition:
 // per-tile sums of r^2 are converted exactly to a two-word fixed-point integer
@@ -84,6 +89,7 @@ struct ProbDesc {
     const float* F;
     const int32_t* node_ptr;
     const int32_t* node_ent;
+    const int2* item_tr;  // per node item: first and last tile touching its nodes (empty: y < x)
     const int32_t* nb_idx;
     const float* nb_wp;
     float* U[2];
@@ -108,6 +114,7 @@ struct ProbState {
     int mode, k, trial, restart, final_eval, cur, dcur, passes;
     float alpha, beta, gnorm, Jtrial;
     double J, gg;
+    double gd;  // g.d of the current direction (Armijo slope)
 };
 
 // Per-problem descriptor fields read by the step decisions, kept in LDS so
@@ -140,7 +147,7 @@ __device__ __forceinline__ void fill_decide_desc(DecideDesc& o, const ProbDesc& 
 struct PlanEntry {
     int flags;
     float alpha, beta;
-    int tile_off;   // prefix of tiles over problems doing an A-step
+    int tile_off;   // prefix of tiles over problems doing a pass
     int chunk_off;  // prefix of node chunks over problems with node work
     int pad[3];
 };
@@ -174,13 +181,14 @@ struct ServeHost {
 // step (written during step s, read right after barrier s, cleared during
 // step s + 2 by block 0).
 struct Workspace {
-    unsigned long long hi[3][kMaxProblems][kSlots];
+    unsigned long long hi[3][kMaxProblems][kSlots];  // fixed-point sum of r^2
     unsigned long long lo[3][kMaxProblems][kSlots];
     int bad[3][kMaxProblems];
     PlanEntry plan[kMaxProblems];  // first step's plan (written by k_init)
     int n_tiles_active, n_chunks_active, any_active, pad0;
     unsigned arrive[kMaxBlocks * kArriveStride];   // last step each block finished (one line per block)
     unsigned release[kMaxBlocks * kArriveStride];  // last step every block finished (one line per block)
+    unsigned tiles_done[kMaxBlocks * kArriveStride];  // last step whose tiles a block finished
     unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
     int stop;                                      // served mode: every block leaves after this barrier
     int err;
@@ -190,6 +198,7 @@ struct Workspace {
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float2 ld2(const float* p, int n) {
     const f2v v = gld(reinterpret_cast<const f2v*>(p))[n];
@@ -436,7 +445,10 @@ __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, cons
     float fx, fy;
     split_coord(px.x, ux, x0, fx);
     split_coord(px.y, uy, y0, fy);
-    const int lx = x0 - g.bx0, ly = y0 - g.by0;
+    // padding lanes of a partly filled chunk (their residual is zeroed) read a
+    // fixed in-box position instead of their dummy coordinate, so they never
+    // take the global-memory sampler
+    const int lx = live ? x0 - g.bx0 : 1, ly = live ? y0 - g.by0 : 1;
     const bool in_box = g.staged && lx >= 1 && ly >= 1 && lx + 2 < g.bw && ly + 2 < g.bh;
     float gv, gx, gy;
     if (in_box)
@@ -504,33 +516,34 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-// Block-level sum of two per-lane values in fixed order: wave sums through
+// Block-level sums of four per-lane values in fixed order: wave sums through
 // DPP, then the 16 wave partials summed across the first row of wave 0
-// (quad_perm, row_half_mirror, row_mirror); lane 0 stores them.
-static_assert(kWavesPerBlock <= 16, "block_store2 folds one 16-lane row");
-__device__ __forceinline__ void block_store2(double a, double b, double* s_red, double* out) {
+// (quad_perm, row_half_mirror, row_mirror); lane 0 stores them to out[0..3].
+static_assert(kWavesPerBlock <= 16, "block_store4 folds one 16-lane row");
+__device__ __forceinline__ void block_store4(const double (&in)[4], double* s_red, double* out) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    a = wave_sum63(a);
-    b = wave_sum63(b);
+    double w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = wave_sum63(in[q]);
     if (lane == kWave - 1) {
-        s_red[2 * wave] = a;
-        s_red[2 * wave + 1] = b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_red[4 * wave + q] = w[q];
     }
     __syncthreads();
     if (wave == 0) {
-        double x = lane < kWavesPerBlock ? s_red[2 * lane] : 0.0;
-        double y = lane < kWavesPerBlock ? s_red[2 * lane + 1] : 0.0;
-        x += dpp_d<0xB1>(x);
-        y += dpp_d<0xB1>(y);
-        x += dpp_d<0x4E>(x);
-        y += dpp_d<0x4E>(y);
-        x += dpp_d<0x141>(x);
-        y += dpp_d<0x141>(y);
-        x += dpp_d<0x140>(x);
-        y += dpp_d<0x140>(y);
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double x = lane < kWavesPerBlock ? s_red[4 * lane + q] : 0.0;
+            x += dpp_d<0xB1>(x);
+            x += dpp_d<0x4E>(x);
+            x += dpp_d<0x141>(x);
+            x += dpp_d<0x140>(x);
+            v[q] = x;
+        }
         if (lane == 0) {
-            st_wt(out, x);
-            st_wt(out + 1, y);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) st_wt(out + q, v[q]);
         }
     }
     __syncthreads();
@@ -569,70 +582,6 @@ __device__ __forceinline__ void node_cache_fill(const ProbDesc* __restrict__ D, 
     nc->ent[threadIdx.x] = ent;
 }
 
-// A-step node item (128 nodes, 8 lanes per node): materialise d, g.d and the
-// spring energy at the candidate (both halves of every ordered pair).  Loads
-// go out in two rounds: the node's own values with the neighbour list, then
-// the neighbours' values (further neighbours, past 8, in later rounds).
-__device__ void node_item_a(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
-                            double* s_red, int par, const NodeCache* nc) {
-    const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
-    const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int sub = lane % kLanesPerNode;
-    const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
-    double gd = 0.0, E = 0.0;
-    if (n < D->nnodes) {
-        const float* U = D->U[cur];
-        const float* G = D->G[cur];
-        const float* Dp = D->D[dcur];
-        const int md = D->nb_maxdeg;
-        int j = -1;
-        float w = 0.f;
-        if (nc) {
-            j = nc->j[threadIdx.x];
-            w = nc->w[threadIdx.x];
-        } else if (sub < md) {
-            j = gld(D->nb_idx)[(size_t)n * md + sub];
-            w = gld(D->nb_wp)[(size_t)n * md + sub];
-        }
-        const float2 un = ld2(U, n);
-        const float2 g = init ? make_float2(0.f, 0.f) : ld2(G, n);
-        const float2 dp = init ? make_float2(0.f, 0.f) : ld2(Dp, n);
-        float2 uj = make_float2(0.f, 0.f), gj = uj, dj = uj;
-        if (j >= 0) {
-            uj = ld2(U, j);
-            if (!init) {
-                gj = ld2(G, j);
-                dj = ld2(Dp, j);
-            }
-        }
-        const float2 d = init ? make_float2(0.f, 0.f) : direction(g, dp, beta, restart);
-        if (sub == 0) {
-            st2_wt(D->D[dcur ^ 1], n, d);
-            gd = (double)g.x * (double)d.x + (double)g.y * (double)d.y;
-        }
-        const float2 ucn = step_to(un, d, alpha);
-        for (int k = sub; k < md; k += kLanesPerNode) {
-            if (k != sub) {
-                j = gld(D->nb_idx)[(size_t)n * md + k];
-                if (j < 0) continue;
-                w = gld(D->nb_wp)[(size_t)n * md + k];
-                uj = ld2(U, j);
-                if (!init) {
-                    gj = ld2(G, j);
-                    dj = ld2(Dp, j);
-                }
-            } else if (j < 0) {
-                continue;
-            }
-            const float2 ucj = init ? uj : step_to(uj, direction(gj, dj, beta, restart), alpha);
-            const float dx = ucn.x - ucj.x, dy = ucn.y - ucj.y;
-            E += (double)(w * (dx * dx + dy * dy));
-        }
-    }
-    block_store2(gd, E, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
-}
-
 // Node pass of a finished served problem: copy the iterate from the plan's
 // buffer U[1] to the caller's buffer U[0].
 __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
@@ -643,24 +592,30 @@ __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
     }
 }
 
-// B-step node item: new point, gradient gather, |g|^2, g.(g - g_prev), and
-// the scatter of (u, g, d) into the tile records for the next A-step.  Loads
-// go out in three rounds: topology and the node's own values; the first tile
-// entry and neighbour values; the entry's tile partials.  Entries and
-// neighbours past the first 8 follow in order.
-__device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, double* s_red,
-                            int par, const NodeCache* nc) {
+// Node item of a pass (8 lanes per node): direction d, trial point
+// u + alpha d, the gradient there (tile partials gathered through the node CSR
+// plus the spring term), written to the other buffers for the case the trial
+// is accepted, and the sums |g|^2, g.(g - g_prev), g.d and the spring energy
+// at the trial point (both halves of every ordered pair, as the reference
+// sums them).  Loads go out in two rounds: the node's own values, its first
+// tile entry and neighbour; then the entry's tile partials.  Entries and
+// neighbours past the first 8 follow in order.  The first evaluation (F_INIT)
+// has d = 0: the trial point is u.
+__device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
+                            double* s_red, int par, const NodeCache* nc) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
+    const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int sub = lane % kLanesPerNode;
     const int n = item * kNodesPerItem + wave * kNodesPerWave + lane / kLanesPerNode;
     const bool valid = n < D->nnodes;
-    double gg = 0.0, gnum = 0.0;
-    float2 un = make_float2(0.f, 0.f), go = make_float2(0.f, 0.f);
+    double gg = 0.0, gnum = 0.0, gdot = 0.0, E = 0.0;
+    float2 un = make_float2(0.f, 0.f), go = make_float2(0.f, 0.f), dn = make_float2(0.f, 0.f);
     double sx = 0.0, sy = 0.0;
     float rx = 0.f, ry = 0.f;
     const float* U = D->U[cur];
-    const float* Dn = D->D[dcur ^ 1];
+    const float* G = D->G[cur];
+    const float* Dp = D->D[dcur];
     if (valid) {
         const int md = D->nb_maxdeg;
         int e0, e1, j = -1, ent;
@@ -680,13 +635,15 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             }
             ent = e0 + sub < e1 ? gld(D->node_ent)[e0 + sub] : -1;
         }
-        const float2 dn = ld2(Dn, n);
         const float2 u0 = ld2(U, n);
-        go = ld2(D->G[cur], n);
+        if (!init) {
+            go = ld2(G, n);
+            dn = direction(go, ld2(Dp, n), beta, restart);
+        }
         float2 uj = make_float2(0.f, 0.f), dj = uj;
         if (j >= 0) {
             uj = ld2(U, j);
-            dj = ld2(Dn, j);
+            if (!init) dj = direction(ld2(G, j), ld2(Dp, j), beta, restart);
         }
         if (ent >= 0) {
             const float2 q = ld2(D->TP, ent);
@@ -698,20 +655,22 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             sx += (double)q.x;
             sy += (double)q.y;
         }
-        un = step_to(u0, dn, alpha);
+        un = init ? u0 : step_to(u0, dn, alpha);
         for (int k = sub; k < md; k += kLanesPerNode) {
             if (k != sub) {
                 j = gld(D->nb_idx)[(size_t)n * md + k];
                 if (j < 0) continue;
                 w = gld(D->nb_wp)[(size_t)n * md + k];
                 uj = ld2(U, j);
-                dj = ld2(Dn, j);
+                if (!init) dj = direction(ld2(G, j), ld2(Dp, j), beta, restart);
             } else if (j < 0) {
                 continue;
             }
-            const float2 ujn = step_to(uj, dj, alpha);
-            rx += w * (un.x - ujn.x);
-            ry += w * (un.y - ujn.y);
+            const float2 ujn = init ? uj : step_to(uj, dj, alpha);
+            const float dx = un.x - ujn.x, dy = un.y - ujn.y;
+            rx += w * dx;
+            ry += w * dy;
+            E += (double)(w * (dx * dx + dy * dy));
         }
     }
     sx = group_sum(sx);
@@ -724,97 +683,85 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
         if (sub == 0) {
             st2_wt(D->U[cur ^ 1], n, un);
             st2_wt(D->G[cur ^ 1], n, make_float2(gx, gy));
+            st2_wt(D->D[dcur ^ 1], n, dn);
             gg = (double)gx * (double)gx + (double)gy * (double)gy;
             gnum = (double)gx * ((double)gx - (double)go.x) + (double)gy * ((double)gy - (double)go.y);
+            gdot = (double)gx * (double)dn.x + (double)gy * (double)dn.y;
         }
     }
-    block_store2(gg, gnum, s_red, D->NCP + 2 * ((size_t)par * D->nchunks + item));
+    const double sums[4] = {gg, gnum, gdot, E};
+    block_store4(sums, s_red, D->NCP + 4 * ((size_t)par * D->nchunks + item));
 }
 
-// State transition of one problem after a step (run identically in every block).  just_did_a:
-// the finished step was an A-step (Armijo decision follows), else a B-step
-// (stop test and beta follow).  fin: node-chunk sums of the finished step.
-__device__ int decide(const DecideDesc& D, ProbState& st, bool just_did_a, double sr2, bool bad, double fin0,
-                      double fin1, bool writer) {
-    int flags = 0;
-    if (st.mode == MODE_DONE) return 0;
-    if (just_did_a) {
-        // B-decision: Armijo test on the trial just evaluated (solver_global_cg.py:419-448)
-        if (st.mode == MODE_PASS || st.mode == MODE_INIT) {
-            const double gd = fin0, E = fin1;
-            double Jc = 0.5 * sr2 * D.invNp + (double)D.alpha_reg * (0.25 * E);
-            if (bad) Jc = NAN;
-            bool accept = false;
-            if (st.mode == MODE_INIT || st.final_eval) {
-                accept = true;
-            } else if (st.trial == 0 && !st.restart && gd >= 0.0) {
-                st.restart = 1;  // non-descent direction: redo the pass with d = -g
-            } else if (Jc <= st.J + 1e-4 * (double)st.alpha * gd) {
-                accept = true;
-            } else {
-                st.Jtrial = (float)Jc;
-                st.trial += 1;
-                st.alpha *= 0.5f;
-                if (st.trial == 10) st.final_eval = 1;  // alpha = 2^-10 is taken untested
-            }
-            if (accept) {
-                flags |= F_DOB;
-                if (st.mode != MODE_INIT) {
-                    if (D.hist && writer) {
-                        D.hist[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
-                        D.hist[3 * st.k + 1] = st.gnorm;
-                        D.hist[3 * st.k + 2] = st.alpha;
-                    }
-                    st.k += 1;
-                }
-                st.J = Jc;
-                st.mode = MODE_NODE;
-            }
-        }
+// State transition of one problem after a pass (run identically in every
+// block): sr2 = sum of r^2 and bad = a non-finite residual (tiles), and the
+// node-item sums at the trial point gg = |g|^2, gnum = g.(g - g_prev),
+// gdot = g.d, E = spring energy.  Returns the flags of the problem's next step.
+__device__ int decide(const DecideDesc& D, ProbState& st, double sr2, bool bad, double gg, double gnum, double gdot,
+                      double E, bool writer) {
+    if (st.mode != MODE_PASS && st.mode != MODE_INIT) return 0;
+    // Armijo test on the trial just evaluated (solver_global_cg.py:419-448)
+    double Jc = 0.5 * sr2 * D.invNp + (double)D.alpha_reg * (0.25 * E);
+    if (bad) Jc = NAN;
+    const bool init = st.mode == MODE_INIT;
+    const bool accept = init || st.final_eval || Jc <= st.J + 1e-4 * (double)st.alpha * st.gd;
+    if (!accept) {
+        st.Jtrial = (float)Jc;
+        st.trial += 1;
+        st.alpha *= 0.5f;
+        if (st.trial == 10) st.final_eval = 1;  // alpha = 2^-10 is taken untested
     } else {
-        // A-decision: stop test on |grad| at the new point, PR+ beta (solver_global_cg.py:369-415)
-        if (st.mode == MODE_NODE) {
-            const double gg = fin0, gnum = fin1;
-            st.cur ^= 1;
-            st.dcur ^= 1;
-            const float gnorm = (float)sqrt(gg);
-            if (st.k >= D.max_iter) {
-                // while_loop exits on k < max_iter before evaluating anything
-                st.mode = MODE_DONE;
-                if (writer) *D.n_iters = st.k;
-            } else if (gnorm < D.tol) {
-                st.mode = MODE_DONE;
-                if (D.hist && writer) {
-                    D.hist[3 * st.k] = (float)st.J;
-                    D.hist[3 * st.k + 1] = gnorm;
-                    D.hist[3 * st.k + 2] = 0.f;
-                }
-                if (writer) *D.n_iters = st.k;
-            } else {
-                float b = 0.f;
-                if (st.k > 0) {
-                    b = (float)gnum / ((float)st.gg + 1e-16f);
-                    if (b < 0.f) b = 0.f;  // max(beta, 0); NaN propagates like jnp.maximum
-                }
-                st.beta = b;
-                st.gg = gg;
-                st.gnorm = gnorm;
-                st.mode = MODE_PASS;
-                st.alpha = 1.f;
-                st.trial = 0;
-                st.restart = 0;
-                st.final_eval = 0;
+        if (!init) {
+            if (D.hist && writer) {
+                D.hist[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
+                D.hist[3 * st.k + 1] = st.gnorm;
+                D.hist[3 * st.k + 2] = st.alpha;
             }
+            st.k += 1;
         }
-        if (st.mode == MODE_PASS || st.mode == MODE_INIT) {
-            flags |= F_DOA;
-            st.passes += 1;
+        st.J = Jc;
+        st.cur ^= 1;  // the trial point, its gradient and d become current
+        st.dcur ^= 1;
+        // stop test on |grad| at the new point, PR+ beta (solver_global_cg.py:364-415)
+        const float gnorm = (float)sqrt(gg);
+        if (st.k >= D.max_iter) {
+            // while_loop exits on k < max_iter before evaluating anything
+            st.mode = MODE_DONE;
+            if (writer) *D.n_iters = st.k;
+        } else if (gnorm < D.tol) {
+            st.mode = MODE_DONE;
+            if (D.hist && writer) {
+                D.hist[3 * st.k] = (float)st.J;
+                D.hist[3 * st.k + 1] = gnorm;
+                D.hist[3 * st.k + 2] = 0.f;
+            }
+            if (writer) *D.n_iters = st.k;
+        } else {
+            float b = 0.f;
+            if (st.k > 0) {
+                b = (float)gnum / ((float)st.gg + 1e-16f);
+                if (b < 0.f) b = 0.f;  // max(beta, 0); NaN propagates like jnp.maximum
+            }
+            // slope of d_new = -g + beta d: g.d_new = -|g|^2 + beta g.d; a
+            // non-descent direction restarts with d = -g (solver_global_cg.py:391-413)
+            const double gdn = st.k > 0 ? -gg + (double)b * gdot : -gg;
+            st.restart = gdn >= 0.0 ? 1 : 0;
+            st.gd = st.restart ? -gg : gdn;
+            st.beta = b;
+            st.gg = gg;
+            st.gnorm = gnorm;
+            st.mode = MODE_PASS;
+            st.alpha = 1.f;
+            st.trial = 0;
+            st.final_eval = 0;
         }
     }
+    if (st.mode == MODE_DONE) return 0;
+    st.passes += 1;
+    int flags = F_DOA | F_DOB;
     if (st.restart) flags |= F_RESTART;
     if (st.cur) flags |= F_CUR;
     if (st.dcur) flags |= F_DCUR;
-    if (st.mode == MODE_INIT) flags |= F_INIT;
     return flags;
 }
 
@@ -830,7 +777,6 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
                              int* s_coff, int* s_hdr, unsigned long long* s_hi, unsigned long long* s_lo,
                              int* s_bad) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const bool just_did_a = (step & 1) == 0;
     const int par = step & 1, buf = step % 3;
     const bool writer = blockIdx.x == 0;
     const unsigned long long admit =
@@ -841,37 +787,39 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
         const int p = __builtin_amdgcn_readfirstlane(pw);
         const DecideDesc& D = s_dd[p];
         const int fl = s_flags[p];
-        const bool had = just_did_a ? (fl & F_DOA) != 0 : (fl & F_DOB) != 0;
+        const bool had = (fl & F_DOA) != 0;
         unsigned long long hi = 0ull, lo = 0ull;
         int bad = 0;
-        if (just_did_a) {
+        // node-chunk sums: chunk c = lane + 64 j added in j order
+        double sg = 0.0, sn = 0.0, sd = 0.0, sE = 0.0;
+        if (had) {
             if (lane < kSlots) {
                 hi = gld(&ws->hi[buf][p][0])[lane];
                 lo = gld(&ws->lo[buf][p][0])[lane];
             }
             if (lane == kWave - 1) bad = gld(&ws->bad[buf][p])[0];
-        }
-        // node-chunk sums: chunk c = lane + 64 j added in j order
-        double a = 0.0, b = 0.0;
-        if (had) {
             const int nch = D.nchunks;
-            const D2IC_GLOBAL d2v* ncp = gld(reinterpret_cast<const d2v*>(D.NCP)) + (size_t)par * nch;
+            const D2IC_GLOBAL d4v* ncp = gld(reinterpret_cast<const d4v*>(D.NCP)) + (size_t)par * nch;
             for (int c0 = 0; c0 < nch; c0 += 4 * kWave) {
-                d2v v[4];
+                d4v v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int c = c0 + u * kWave + lane;
-                    v[u] = c < nch ? ncp[c] : d2v{0.0, 0.0};
+                    v[u] = c < nch ? ncp[c] : d4v{0.0, 0.0, 0.0, 0.0};
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    a += v[u].x;
-                    b += v[u].y;
+                    sg += v[u].x;
+                    sn += v[u].y;
+                    sd += v[u].z;
+                    sE += v[u].w;
                 }
             }
         }
-        a = wave_sum63(a);
-        b = wave_sum63(b);
+        sg = wave_sum63(sg);
+        sn = wave_sum63(sn);
+        sd = wave_sum63(sd);
+        sE = wave_sum63(sE);
         hi = wave_sum63(hi);
         lo = wave_sum63(lo);
         if (lane == kWave - 1) {
@@ -885,7 +833,7 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
                 fl2 = 0;
                 finished = true;
             } else {
-                fl2 = decide(D, st, just_did_a, sr2, bad != 0, a, b, writer);
+                fl2 = decide(D, st, sr2, bad != 0, sg, sn, sd, sE, writer);
                 if (prev != MODE_DONE && st.mode == MODE_DONE) {
                     if (sh && st.cur) {
                         st.mode = MODE_COPY;
@@ -896,15 +844,14 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
                 }
             }
             if (sh && ((admit >> p) & 1ull)) {
-                // a served request joins: first evaluation at u0 on the next A-step
+                // a served request joins: first evaluation at u0 in the next step
                 ProbState z;
                 z.mode = MODE_INIT;
                 z.k = z.trial = z.restart = z.final_eval = z.cur = z.dcur = 0;
                 z.alpha = z.beta = z.gnorm = z.Jtrial = 0.f;
-                z.J = z.gg = 0.0;
-                const bool next_is_a = just_did_a ? false : true;
-                z.passes = next_is_a ? 1 : 0;
-                fl2 = next_is_a ? (F_DOA | F_INIT) : 0;
+                z.J = z.gg = z.gd = 0.0;
+                z.passes = 1;
+                fl2 = F_DOA | F_DOB | F_INIT;
                 st = z;
                 finished = false;
             }
@@ -935,7 +882,7 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
         for (int p = threadIdx.x; p < P; p += kBlock)
             __hip_atomic_store(&ws->bad[zb][p], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // block-local residual accumulators of the next A-step
+    // block-local residual accumulators of the next step
     for (int p = threadIdx.x; p < P; p += kBlock) {
         s_hi[p] = 0ull;
         s_lo[p] = 0ull;
@@ -948,8 +895,8 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
             s_toff[p] = to;
             s_coff[p] = co;
             if (fl & F_DOA) to += s_dd[p].ntiles;
-            if (fl & (F_DOA | F_DOB | F_COPY)) co += s_dd[p].nchunks;
-            if (s_st[p].mode != MODE_DONE) any = 1;  // a problem whose trial was rejected idles one step
+            if (fl & (F_DOB | F_COPY)) co += s_dd[p].nchunks;
+            if (s_st[p].mode != MODE_DONE) any = 1;
         }
         s_hdr[0] = to;
         s_hdr[1] = co;
@@ -1047,6 +994,73 @@ __device__ __forceinline__ int item_begin(int vb, int nC, int nb) {
 constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
 constexpr unsigned kBarrierMinPolls = 1u << 20;
 
+// Virtual block holding global tile gt: wave w takes tiles
+// [nT w / nwaves, nT (w + 1) / nwaves), so gt belongs to
+// w = ceil((gt + 1) nwaves / nT) - 1.
+__device__ __forceinline__ int tile_owner_vb(int gt, int nT, int nwaves) {
+    const int w = (int)(((long long)(gt + 1) * nwaves + nT - 1) / nT) - 1;
+    return w / kWavesPerBlock;
+}
+
+// Hardware block of virtual block v (inverse of the XCD-aware numbering).
+__device__ __forceinline__ int block_of_vb(int v, int nb) {
+    if (nb % 8 == 0) {
+        const int per = nb / 8;
+        return (v % per) * 8 + v / per;
+    }
+    return v;
+}
+
+// Block-wide: wait until every block holding a tile that touches the nodes of
+// node item `item` (tile range item_tr, offset toff in this step's global tile
+// order) has published this step's tile partials, then acquire them.  The
+// wave-0 poll is bounded like the grid barrier; on expiry (or another block's
+// error) it raises ws->err, clears *s_ok and returns false.
+__device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, int nT, int nb, int nwaves, int step,
+                           Workspace* __restrict__ ws, int* s_ok) {
+    const int* tr = reinterpret_cast<const int*>(D->item_tr) + 2 * (size_t)item;
+    const int tlo = gld(tr)[0], thi = gld(tr)[1];
+    if (thi < tlo || nT <= 0) return true;
+    const int va = tile_owner_vb(toff + tlo, nT, nwaves), vz = tile_owner_vb(toff + thi, nT, nwaves);
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        const uint64_t tstart = wall_clock64();
+        unsigned polls = 0;
+        bool ok = true;
+        while (true) {
+            bool mine = true;
+            for (int v0 = va; v0 <= vz; v0 += kWave) {
+                const int v = v0 + lane;
+                if (v <= vz)
+                    mine = mine && __hip_atomic_load(&ws->tiles_done[block_of_vb(v, nb) * kArriveStride],
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                                       (unsigned)(step + 1);
+            }
+            if (__all(mine)) break;
+            if (__hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = false;
+                break;
+            }
+            ++polls;
+            if (polls > kBarrierMinPolls && wall_clock64() - tstart > kBarrierTimeoutTicks) {
+                if (lane == 0) {
+                    atomicExch(&ws->err, 1);
+                    printf("d2ic: tile wait timeout (block %u, step %d)\n", blockIdx.x, step);
+                }
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (lane == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (!ok) *s_ok = 0;
+        }
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
 // sh == nullptr: a fixed batch of P problems (d2ic_cg_solve_batch).  Otherwise a
 // served session over P <= kMaxSlots slots whose requests arrive through the
 // mailbox; the kernel runs until it has been idle for sh->idle_ticks or the
@@ -1081,7 +1095,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ int s_hdr[3];
     __shared__ int s_ok;
     __shared__ int s_stop;
-    __shared__ double s_red[2 * kWavesPerBlock];
+    __shared__ double s_red[4 * kWavesPerBlock];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
     __shared__ typename std::conditional<kSingle, NodeCache, int>::type s_nc_store;
     NodeCache* s_nc = kSingle ? reinterpret_cast<NodeCache*>(&s_nc_store) : nullptr;
@@ -1121,6 +1135,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         s_hdr[0] = ws->n_tiles_active;
         s_hdr[1] = ws->n_chunks_active;
         s_hdr[2] = ws->any_active;
+        s_ok = 1;
     }
     // single problem: the block's node item is the same in every step that has
     // node work (items [ceil(vb nC / nb), ...) with nC = nchunks); cache its
@@ -1140,29 +1155,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     uint64_t idle_t0 = wall_clock64();     // served mode, master wave: start of the current idle spell
 
     for (int step = 0; step < max_steps; ++step) {
-        const bool isA = (step & 1) == 0;
         const int par = step & 1, buf = step % 3;
         if (!sh && !s_hdr[2]) break;
         const int nT = s_hdr[0], nC = s_hdr[1];
 
-        // ---- node items (A: d, g.d, spring energy; B: gradient, dots): block vb
-        //      takes items [ceil(vb nC / nb), ceil((vb + 1) nC / nb)) ----
-        const int ci0 = item_begin(vb, nC, nb), ci1 = item_begin(vb + 1, nC, nb);
-        for (int c = ci0; c < ci1; ++c) {
-            const int p = kSingle ? 0 : find_prob(s_coff, P, c);
-            const ProbDesc* D = descs + p;
-            const int fl = s_flags[p];
-            if (fl & F_COPY)
-                node_item_copy(D, c - s_coff[p]);
-            else if (isA)
-                node_item_a(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par, c == nc_item ? s_nc : nullptr);
-            else
-                node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_red, par, c == nc_item ? s_nc : nullptr);
-        }
-
-        // ---- tiles (A only): contiguous share per wave, software-pipelined:
-        //      record 2 tiles ahead, staged box 1 tile ahead, pixels 1 chunk ahead ----
-        if (isA && nT > 0) {
+        // ---- tiles: contiguous share per wave, software-pipelined: record 2
+        //      tiles ahead, staged box 1 tile ahead, pixels 1 chunk ahead ----
+        const int my_t0 = (int)(((long long)nT * vb * kWavesPerBlock) / nwaves);
+        const int my_t1 = (int)(((long long)nT * (vb + 1) * kWavesPerBlock) / nwaves);
+        if (nT > 0) {
             const int t0 = (int)(((long long)nT * gwave) / nwaves);
             const int t1 = (int)(((long long)nT * (gwave + 1)) / nwaves);
             if (t0 < t1) {
@@ -1264,13 +1265,38 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             }
         }
         __syncthreads();
-        if (isA) {
+        {
             const int slot = blockIdx.x % kSlots;
             for (int p = threadIdx.x; p < P; p += kBlock) {
                 if (s_hi[p]) atomicAdd(&ws->hi[buf][p][slot], s_hi[p]);
                 if (s_lo[p]) atomicAdd(&ws->lo[buf][p][slot], s_lo[p]);
                 if (s_bad[p]) atomicOr(&ws->bad[buf][p], 1);
             }
+        }
+
+        // ---- publish that this block's tiles are done (their partials
+        //      released), then node items: block vb takes items
+        //      [ceil(vb nC / nb), ceil((vb + 1) nC / nb)); an item first waits
+        //      for the blocks holding the tiles of its nodes ----
+        const int ci0 = item_begin(vb, nC, nb), ci1 = item_begin(vb + 1, nC, nb);
+        if (nC > 0) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (my_t0 < my_t1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(&ws->tiles_done[blockIdx.x * kArriveStride], (unsigned)(step + 1),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        for (int c = ci0; c < ci1; ++c) {
+            const int p = kSingle ? 0 : find_prob(s_coff, P, c);
+            const ProbDesc* D = descs + p;
+            const int fl = s_flags[p];
+            if (fl & F_COPY) {
+                node_item_copy(D, c - s_coff[p]);
+                continue;
+            }
+            if (!wait_tiles(D, c - s_coff[p], s_toff[p], nT, nb, nwaves, step, ws, &s_ok)) break;
+            node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par, c == nc_item ? s_nc : nullptr);
         }
 
         // ---- grid barrier: every block publishes its arrival; block 0's first
@@ -1284,11 +1310,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             // the loads of the blocks still working on that XCD.  (An idle
             // block's loads of this step have all returned: their values were
             // used before the block arrived.)
-            const bool wrote = blockIdx.x == 0 || (isA && nT > 0) || ci0 < ci1;
+            const bool wrote = blockIdx.x == 0 || my_t0 < my_t1 || ci0 < ci1;
             if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-            s_ok = 1;
             s_stop = 0;
         }
         if (blockIdx.x == 0) {
@@ -1418,6 +1443,7 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
         st.Jtrial = 0.f;
         st.J = 0.0;
         st.gg = 0.0;
+        st.gd = 0.0;
         S[p] = st;
         *D.n_iters = -1;
         int to = 0, co = 0;
@@ -1426,7 +1452,7 @@ __global__ void k_init(const ProbDesc* __restrict__ descs, int P, ProbState* __r
             co += descs[q].nchunks;
         }
         PlanEntry e;
-        e.flags = F_DOA | F_INIT;
+        e.flags = F_DOA | F_DOB | F_INIT;
         e.alpha = 0.f;
         e.beta = 0.f;
         e.tile_off = to;
@@ -1449,7 +1475,7 @@ __global__ void k_serve_init(ProbState* __restrict__ S, int P) {
     st.mode = MODE_DONE;
     st.k = st.trial = st.restart = st.final_eval = st.cur = st.dcur = st.passes = 0;
     st.alpha = st.beta = st.gnorm = st.Jtrial = 0.f;
-    st.J = st.gg = 0.0;
+    st.J = st.gg = st.gd = 0.0;
     S[p] = st;
 }
 
@@ -1583,6 +1609,24 @@ __global__ void k_tiles(const int32_t* __restrict__ gstart, const int32_t* __res
         tcnt[t] = min(kTile, s1 - s);
         tn[t] = nodes;
     }
+}
+
+// Per node item (kNodesPerItem consecutive nodes): the first and last tile
+// holding a pixel of one of its nodes, from the node -> (tile, slot) table;
+// (INT_MAX, -1) when no tile touches the item.
+__global__ void k_item_tiles(const int32_t* __restrict__ node_ptr, const int32_t* __restrict__ node_ent, int64_t nn,
+                             int64_t nitems, int2* __restrict__ out) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nitems) return;
+    int lo = INT_MAX, hi = -1;
+    const int64_t n1 = min((c + 1) * kNodesPerItem, nn);
+    for (int64_t n = c * kNodesPerItem; n < n1; ++n)
+        for (int e = node_ptr[n]; e < node_ptr[n + 1]; ++e) {
+            const int t = node_ent[e] >> 2;
+            lo = min(lo, t);
+            hi = max(hi, t);
+        }
+    out[c] = make_int2(lo, hi);
 }
 
 // Tile records start zeroed with the pixel count in slot 24 and the static
@@ -1777,6 +1821,7 @@ ProbDesc make_desc(const d2ic_cg_plan* pl, const float* def_image, int32_t H, in
     d.F = pl->F;
     d.node_ptr = pl->node_ptr;
     d.node_ent = pl->node_ent;
+    d.item_tr = pl->item_tr;
     d.nb_idx = pl->nb_idx;
     d.nb_wp = pl->nb_wp;
     d.U[0] = u;
@@ -1877,7 +1922,8 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
     PA(pl->G[1], 2 * n_nodes);
     PA(pl->D[0], 2 * n_nodes);
     PA(pl->D[1], 2 * n_nodes);
-    PA(pl->NCP, 4 * pl->nchunks);  // 2 doubles per chunk, double-buffered by step parity
+    PA(pl->NCP, 8 * pl->nchunks);  // per node chunk: |g|^2, g.(g - g_prev), g.d, E; by step parity
+    PA(pl->item_tr, pl->nchunks);
     PA(pl->node_ptr, n_nodes + 1);
     PA(pl->nb_idx, n_nodes * nb_max_degree);
     PA(pl->nb_wp, n_nodes * nb_max_degree);
@@ -2013,6 +2059,12 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PH(hipcub::DeviceRadixSort::SortPairs(tmp3.p, b5, ek.as<int32_t>(), eks.as<int32_t>(), ev.as<int32_t>(),
                                               pl->node_ent, (int)ne, 0, bits, st));
         PH(hipcub::DeviceScan::ExclusiveSum(tmp3.p, b6, cnt.as<int32_t>(), pl->node_ptr, (int)(n_nodes + 1), st));
+        PH(hipStreamSynchronize(st));
+    }
+    if (pl->nchunks > 0) {
+        k_item_tiles<<<ceil_div(pl->nchunks, tb), tb, 0, st>>>(pl->node_ptr, pl->node_ent, n_nodes, pl->nchunks,
+                                                               pl->item_tr);
+        PH(hipGetLastError());
         PH(hipStreamSynchronize(st));
     }
 #undef PA
@@ -2207,7 +2259,7 @@ int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, Pro
 constexpr int kNoAllocProblems = 8;
 constexpr size_t kScratchOffS = ((sizeof(ProbDesc) * kNoAllocProblems + 255) / 256) * 256;
 constexpr size_t kScratchOffW = kScratchOffS + ((sizeof(ProbState) * kNoAllocProblems + 255) / 256) * 256;
-constexpr size_t kScratchBytes = kScratchOffW + ((sizeof(Workspace) + 255) / 256) * 256;  // ~0.47 MB
+constexpr size_t kScratchBytes = kScratchOffW + ((sizeof(Workspace) + 255) / 256) * 256;  // ~0.6 MB
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 

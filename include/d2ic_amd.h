@@ -118,7 +118,7 @@ int d2ic_cg_solve_batch(int32_t n_problems, d2ic_cg_plan* const* plans, const fl
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
  * kernels and results (bitwise) as d2ic_cg_solve_batch, for at most 8
  * problems: the per-launch descriptors, solver states and workspace live in
- * one static device array of ~0.47 MB per GPU (held until the process ends)
+ * one static device array of ~0.6 MB per GPU (held until the process ends)
  * instead of a hipMallocAsync block, and descriptors travel as a kernel
  * argument instead of through pinned staging memory.  More than 8 problems
  * is rejected with D2IC_ERR_INVALID.  Every call on a GPU shares that GPU's
