@@ -7,7 +7,8 @@
 //     (f at the pixel);
 //   * per tile: tile_cnt, tile_nodes (its element's 4 node ids), TV (static
 //     record, kTV ints: [0] pixel count, [1..4] floor(min x) - 1, floor(max x),
-//     floor(min y) - 1, floor(max y));
+//     floor(min y) - 1, floor(max y), [5] 1 if every pixel sits at an exact
+//     pixel centre (j + 0.5, i + 0.5));
 //   * node -> (tile, local node) gather table: node_ptr (nn + 1), node_ent
 //     entries (t << 2) | a;
 //   * spring stencil: nb_idx (nn x nb_maxdeg, -1 padded) and nb_wp = the

@@ -284,7 +284,7 @@ __device__ __forceinline__ int tile_ids(const ProbDesc* __restrict__ D, int t, i
         const int a = (lane & 7) >> 1;
         return a == 0 ? nd.x : (a == 1 ? nd.y : (a == 2 ? nd.z : nd.w));
     }
-    if (lane < 29) return __float_as_int(gld(D->TV)[(size_t)t * kTV + (lane - 24)]);
+    if (lane < 30) return __float_as_int(gld(D->TV)[(size_t)t * kTV + (lane - 24)]);
     return 0;
 }
 
@@ -337,6 +337,7 @@ __device__ __forceinline__ void tile_uc(float tv, int flags, float alpha, float 
 struct TileBox {
     int cnt, bx0, by0, bw, bh;
     bool staged;
+    bool fast;  // staged and every pixel at an exact pixel centre: every sample lies in the box
 };
 
 __device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
@@ -360,12 +361,14 @@ __device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
         g.bh = by1 - g.by0 + 1;
         g.staged = g.bw * g.bh <= kBoxMax;
     }
+    g.fast = g.staged && __float_as_int(bcast(tv, 29)) != 0;
     g.cnt = __builtin_amdgcn_readfirstlane(g.cnt);
     g.bx0 = __builtin_amdgcn_readfirstlane(g.bx0);
     g.by0 = __builtin_amdgcn_readfirstlane(g.by0);
     g.bw = __builtin_amdgcn_readfirstlane(g.bw);
     g.bh = __builtin_amdgcn_readfirstlane(g.bh);
     g.staged = __builtin_amdgcn_readfirstlane((int)g.staged) != 0;
+    g.fast = __builtin_amdgcn_readfirstlane((int)g.fast) != 0;
     return g;
 }
 
@@ -431,7 +434,14 @@ __device__ __forceinline__ void box_store(const TileBox& g, int lane, const floa
 // accumulated per lane into r^2 (FP64) and the 8 gradient partials
 // acc[2a + {0,1}] += r * (dg/dx, dg/dy) * N_a.  Samples inside the staged box
 // read LDS; others read global memory (same values).  Dead lanes add zeros.
-template <int kInterp>
+// kFast (tiles whose pixels all sit at exact pixel centres j + 0.5, i + 0.5,
+// with a staged box): the sample cell is the pixel index plus floor(u), the
+// same value split_coord gives ((x - floor x - 0.5) + u = u exactly), and it
+// lies in the box by construction (u at a pixel is a convex combination of
+// the 4 node values whose range the box covers, with one cell of margin for
+// rounding); the box indices are clamped so no read can leave the box.
+// Otherwise each lane checks the box and falls back to global memory.
+template <int kInterp, bool kFast>
 __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, const TilePix& px,
                                             const float2 (&uc)[4], const TileBox& g, bool live,
                                             const __attribute__((address_space(3))) float* box, float (&acc)[8],
@@ -441,20 +451,28 @@ __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, cons
     const float4 N = px.N;
     const float ux = N.x * uc[0].x + N.y * uc[1].x + N.z * uc[2].x + N.w * uc[3].x;
     const float uy = N.x * uc[0].y + N.y * uc[1].y + N.z * uc[2].y + N.w * uc[3].y;
-    int x0, y0;
-    float fx, fy;
-    split_coord(px.x, ux, x0, fx);
-    split_coord(px.y, uy, y0, fy);
-    // padding lanes of a partly filled chunk (their residual is zeroed) read a
-    // fixed in-box position instead of their dummy coordinate, so they never
-    // take the global-memory sampler
-    const int lx = live ? x0 - g.bx0 : 1, ly = live ? y0 - g.by0 : 1;
-    const bool in_box = g.staged && lx >= 1 && ly >= 1 && lx + 2 < g.bw && ly + 2 < g.bh;
     float gv, gx, gy;
-    if (in_box)
+    if constexpr (kFast) {
+        const float flx = floorf(ux), fly = floorf(uy);
+        const float fx = ux - flx, fy = uy - fly;
+        const int lx = min(max((int)px.x + (int)flx - g.bx0, 1), g.bw - 3);
+        const int ly = min(max((int)px.y + (int)fly - g.by0, 1), g.bh - 3);
         sample_box(interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
-    else
-        sample_at(interp, D->img, D->H, D->W, x0, fx, y0, fy, gv, gx, gy);
+    } else {
+        int x0, y0;
+        float fx, fy;
+        split_coord(px.x, ux, x0, fx);
+        split_coord(px.y, uy, y0, fy);
+        // padding lanes of a partly filled chunk (their residual is zeroed) read a
+        // fixed in-box position instead of their dummy coordinate, so they never
+        // take the global-memory sampler
+        const int lx = live ? x0 - g.bx0 : 1, ly = live ? y0 - g.by0 : 1;
+        const bool in_box = g.staged && lx >= 1 && ly >= 1 && lx + 2 < g.bw && ly + 2 < g.bh;
+        if (in_box)
+            sample_box(interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
+        else
+            sample_at(interp, D->img, D->H, D->W, x0, fx, y0, fy, gv, gx, gy);
+    }
     const float r = live ? gv - f : 0.f;
     r2 += (double)r * (double)r;
     const float qx = r * gx, qy = r * gy;
@@ -1224,16 +1242,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     double r2 = 0.0;
                     const int nch = (b_cur.cnt + kWave - 1) / kWave;
-                    for (int c = 0; c < nch; ++c) {
-                        TilePix pn;
-                        if (c + 1 < nch)
-                            pn = tile_pix(Da, ta, c + 1, lane);
-                        else if (it + 1 < t1)
-                            pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
-                        chunk_accum<kInterp>(Da, px, uc, b_cur,
-                                    c * kWave + lane < b_cur.cnt, my_box, acc, r2);
-                        px = pn;
-                    }
+                    auto chunks = [&](auto fast) {
+                        for (int c = 0; c < nch; ++c) {
+                            TilePix pn;
+                            if (c + 1 < nch)
+                                pn = tile_pix(Da, ta, c + 1, lane);
+                            else if (it + 1 < t1)
+                                pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
+                            chunk_accum<kInterp, decltype(fast)::value>(Da, px, uc, b_cur,
+                                                                         c * kWave + lane < b_cur.cnt, my_box, acc,
+                                                                         r2);
+                            px = pn;
+                        }
+                    };
+                    if (b_cur.fast)
+                        chunks(std::true_type{});
+                    else
+                        chunks(std::false_type{});
                     reduce8_store(acc, lane, gst(Da->TP + 8 * (size_t)ta));
                     r2 = wave_sum63(r2);
                     if (lane == kWave - 1) {
@@ -1638,13 +1663,16 @@ __global__ void k_tv_static(const int32_t* __restrict__ tcnt, int64_t nt, const 
     for (int k = 0; k < kTV; ++k) TV[t * kTV + k] = 0.f;
     const int cnt = tcnt[t];
     float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    int centres = 1;
     for (int l = 0; l < cnt; ++l) {
         const float x = px[t * kTile + l], y = py[t * kTile + l];
         xmin = fminf(xmin, x);
         xmax = fmaxf(xmax, x);
         ymin = fminf(ymin, y);
         ymax = fmaxf(ymax, y);
+        if (!(x >= 0.f && y >= 0.f && x - floorf(x) == 0.5f && y - floorf(y) == 0.5f)) centres = 0;
     }
+    TV[t * kTV + 5] = __int_as_float(centres);
     TV[t * kTV + 0] = __int_as_float(cnt);
     TV[t * kTV + 1] = __int_as_float((int)floorf(xmin) - 1);
     TV[t * kTV + 2] = __int_as_float((int)floorf(xmax));
