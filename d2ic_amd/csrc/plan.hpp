@@ -8,7 +8,11 @@
 //   * per tile: tile_cnt, tile_nodes (its element's 4 node ids), TV (static
 //     record, kTV ints: [0] pixel count, [1..4] floor(min x) - 1, floor(max x),
 //     floor(min y) - 1, floor(max y), [5] 1 if every pixel sits at an exact
-//     pixel centre (j + 0.5, i + 0.5));
+//     pixel centre (j + 0.5, i + 0.5)), TR (kTR words: col0, row0, log2 of
+//     the row width, ax, bx, ay, by, regular flag -- a regular tile's pixels
+//     fill rows of 2^k columns from (col0, row0) and their shape values are
+//     exactly the Q4 functions of xi = ax col + bx, eta = ay row + by, so the
+//     pixel pass loads only F for it);
 //   * node -> (tile, local node) gather table: node_ptr (nn + 1), node_ent
 //     entries (t << 2) | a;
 //   * spring stencil: nb_idx (nn x nb_maxdeg, -1 padded) and nb_wp = the
@@ -23,6 +27,7 @@
 namespace d2ic {
 constexpr int kTile = 256;  // pixels per tile (one element, or a 256-pixel run of one)
 constexpr int kTV = 8;
+constexpr int kTR = 8;  // regular-geometry record per tile
 }  // namespace d2ic
 
 struct d2ic_cg_plan {
@@ -44,6 +49,7 @@ struct d2ic_cg_plan {
     float* D[2] = {nullptr, nullptr};
     float* TP = nullptr;
     float* TV = nullptr;
+    float* TR = nullptr;  // per tile: col0, row0, log2 rw, ax, bx, ay, by, regular flag
     double* NCP = nullptr;
     int2* item_tr = nullptr;  // per node item: first and last tile touching its nodes
     int ref_H = 0, ref_W = 0, interp = 0;

@@ -97,6 +97,7 @@ struct ProbDesc {
     float* D[2];
     float* TP;
     float* TV;
+    const float* TR;  // per tile: regular-geometry record (see k_tile_regular)
     double* NCP;
     const float* img;
     float* hist;
@@ -271,6 +272,32 @@ struct TilePix {  // per-lane pixel data of one chunk (loaded one chunk ahead)
     float4 N;
 };
 
+// Regular tiles (k_tile_regular): pixel i of the tile sits at column
+// col0 + i % rw, row row0 + i / rw (rw a power of two), at the exact pixel
+// centre, and its shape values are the Q4 functions of xi = ax col + bx,
+// eta = ay row + by -- checked bit for bit against the stored values when the
+// plan is built, so computing them gives the same numbers as loading them.
+struct TileGeo {
+    int col0, row0, sh;
+    float ax, bx, ay, by;
+    bool regular;
+};
+
+__device__ __forceinline__ float4 q4_shape(float xi, float eta) {
+    float4 N;
+    N.x = 0.25f * (1.f - xi) * (1.f - eta);
+    N.y = 0.25f * (1.f + xi) * (1.f - eta);
+    N.z = 0.25f * (1.f + xi) * (1.f + eta);
+    N.w = 0.25f * (1.f - xi) * (1.f + eta);
+    return N;
+}
+
+__device__ __forceinline__ void regular_pixel(const TileGeo& g, int i, int& col, int& row, float4& N) {
+    col = g.col0 + (i & ((1 << g.sh) - 1));
+    row = g.row0 + (i >> g.sh);
+    N = q4_shape(__fmaf_rn(g.ax, (float)col, g.bx), __fmaf_rn(g.ay, (float)row, g.by));
+}
+
 // A tile's wave-wide record, one value per lane: lane l < 24 holds component
 // l % 2 of local node (l % 8) / 2 of u (l < 8), g (8 <= l < 16) or d_prev
 // (16 <= l < 24); lanes 24..28 the static record.  Built in two dependent
@@ -285,6 +312,7 @@ __device__ __forceinline__ int tile_ids(const ProbDesc* __restrict__ D, int t, i
         return a == 0 ? nd.x : (a == 1 ? nd.y : (a == 2 ? nd.z : nd.w));
     }
     if (lane < 30) return __float_as_int(gld(D->TV)[(size_t)t * kTV + (lane - 24)]);
+    if (lane < 38) return __float_as_int(gld(D->TR)[(size_t)t * kTR + (lane - 30)]);
     return 0;
 }
 
@@ -309,6 +337,11 @@ __device__ __forceinline__ TilePix tile_pix(const ProbDesc* __restrict__ D, int 
     const f4v nv = gld(reinterpret_cast<const f4v*>(D->N4 + t0))[i];
     r.N = make_float4(nv.x, nv.y, nv.z, nv.w);
     return r;
+}
+
+// f only (regular tiles compute the rest)
+__device__ __forceinline__ float tile_f(const ProbDesc* __restrict__ D, int t, int c, int lane) {
+    return gld(D->F + (size_t)t * kTile)[c * kWave + lane];
 }
 
 __device__ __forceinline__ float bcast(float v, int l) {
@@ -337,7 +370,9 @@ __device__ __forceinline__ void tile_uc(float tv, int flags, float alpha, float 
 struct TileBox {
     int cnt, bx0, by0, bw, bh;
     bool staged;
-    bool fast;  // staged and every pixel at an exact pixel centre: every sample lies in the box
+    bool fast;     // staged and every pixel at an exact pixel centre: every sample lies in the box
+    bool regular;  // fast and a regular tile: pixel position and shape values follow from geo
+    TileGeo geo;
 };
 
 __device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
@@ -362,6 +397,14 @@ __device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
         g.staged = g.bw * g.bh <= kBoxMax;
     }
     g.fast = g.staged && __float_as_int(bcast(tv, 29)) != 0;
+    g.regular = g.fast && __float_as_int(bcast(tv, 37)) != 0;
+    g.geo.col0 = __float_as_int(bcast(tv, 30));
+    g.geo.row0 = __float_as_int(bcast(tv, 31));
+    g.geo.sh = __float_as_int(bcast(tv, 32));
+    g.geo.ax = bcast(tv, 33);
+    g.geo.bx = bcast(tv, 34);
+    g.geo.ay = bcast(tv, 35);
+    g.geo.by = bcast(tv, 36);
     g.cnt = __builtin_amdgcn_readfirstlane(g.cnt);
     g.bx0 = __builtin_amdgcn_readfirstlane(g.bx0);
     g.by0 = __builtin_amdgcn_readfirstlane(g.by0);
@@ -369,6 +412,7 @@ __device__ __forceinline__ TileBox tile_box(float tv, const float2 (&uc)[4]) {
     g.bh = __builtin_amdgcn_readfirstlane(g.bh);
     g.staged = __builtin_amdgcn_readfirstlane((int)g.staged) != 0;
     g.fast = __builtin_amdgcn_readfirstlane((int)g.fast) != 0;
+    g.regular = __builtin_amdgcn_readfirstlane((int)g.regular) != 0;
     return g;
 }
 
@@ -445,7 +489,7 @@ template <int kInterp, bool kFast>
 __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, const TilePix& px,
                                             const float2 (&uc)[4], const TileBox& g, bool live,
                                             const __attribute__((address_space(3))) float* box, float (&acc)[8],
-                                            double& r2) {
+                                            double& r2, int col, int row) {
     const int interp = kInterp >= 0 ? kInterp : D->interp;
     const float f = px.f;
     const float4 N = px.N;
@@ -455,8 +499,8 @@ __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, cons
     if constexpr (kFast) {
         const float flx = floorf(ux), fly = floorf(uy);
         const float fx = ux - flx, fy = uy - fly;
-        const int lx = min(max((int)px.x + (int)flx - g.bx0, 1), g.bw - 3);
-        const int ly = min(max((int)px.y + (int)fly - g.by0, 1), g.bh - 3);
+        const int lx = min(max(col + (int)flx - g.bx0, 1), g.bw - 3);
+        const int ly = min(max(row + (int)fly - g.by0, 1), g.bh - 3);
         sample_box(interp, box, g.bw, lx, fx, ly, fy, gv, gx, gy);
     } else {
         int x0, y0;
@@ -1242,23 +1286,56 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     double r2 = 0.0;
                     const int nch = (b_cur.cnt + kWave - 1) / kWave;
+                    // pixels of the next tile's first chunk: f alone when that tile is regular
+                    auto next_tile_pix = [&]() {
+                        TilePix pn;
+                        if (b_nxt.regular)
+                            pn.f = tile_f(descs + pb, it + 1 - s_toff[pb], 0, lane);
+                        else
+                            pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
+                        return pn;
+                    };
                     auto chunks = [&](auto fast) {
                         for (int c = 0; c < nch; ++c) {
                             TilePix pn;
                             if (c + 1 < nch)
                                 pn = tile_pix(Da, ta, c + 1, lane);
                             else if (it + 1 < t1)
-                                pn = tile_pix(descs + pb, it + 1 - s_toff[pb], 0, lane);
+                                pn = next_tile_pix();
                             chunk_accum<kInterp, decltype(fast)::value>(Da, px, uc, b_cur,
                                                                          c * kWave + lane < b_cur.cnt, my_box, acc,
-                                                                         r2);
+                                                                         r2, (int)px.x, (int)px.y);
                             px = pn;
                         }
                     };
-                    if (b_cur.fast)
+                    if (b_cur.regular) {
+                        // column, xi and the two xi factors of the shape values are the
+                        // same in every chunk (64 is a multiple of the row width)
+                        const TileGeo& gg = b_cur.geo;
+                        const int col = gg.col0 + (lane & ((1 << gg.sh) - 1));
+                        const float xi = __fmaf_rn(gg.ax, (float)col, gg.bx);
+                        const float qa = 0.25f * (1.f - xi), qb = 0.25f * (1.f + xi);
+                        const int rstep = kWave >> gg.sh;
+                        int row = gg.row0 + (lane >> gg.sh);
+                        for (int c = 0; c < nch; ++c) {
+                            TilePix pn;
+                            if (c + 1 < nch)
+                                pn.f = tile_f(Da, ta, c + 1, lane);
+                            else if (it + 1 < t1)
+                                pn = next_tile_pix();
+                            const float eta = __fmaf_rn(gg.ay, (float)row, gg.by);
+                            const float em = 1.f - eta, ep = 1.f + eta;
+                            px.N = make_float4(qa * em, qb * em, qb * ep, qa * ep);
+                            chunk_accum<kInterp, true>(Da, px, uc, b_cur, c * kWave + lane < b_cur.cnt, my_box, acc,
+                                                       r2, col, row);
+                            px = pn;
+                            row += rstep;
+                        }
+                    } else if (b_cur.fast) {
                         chunks(std::true_type{});
-                    else
+                    } else {
                         chunks(std::false_type{});
+                    }
                     reduce8_store(acc, lane, gst(Da->TP + 8 * (size_t)ta));
                     r2 = wave_sum63(r2);
                     if (lane == kWave - 1) {
@@ -1680,6 +1757,73 @@ __global__ void k_tv_static(const int32_t* __restrict__ tcnt, int64_t nt, const 
     TV[t * kTV + 4] = __int_as_float((int)floorf(ymax));
 }
 
+// Regular-geometry record of each tile (one block of kTile threads per tile):
+// the tile is regular when its pixels fill rows of rw (a power of two, 2..64)
+// from (col0, row0) in row-major order at exact pixel centres, and every
+// stored shape value equals q4_shape(ax col + bx, ay row + by) bit for bit,
+// with ax, bx, ay, by read off the first pixels (xi = 2 (N1 + N2) - 1,
+// eta = 2 (N2 + N3) - 1 on an axis-aligned rectangle).  Record:
+// [col0, row0, log2 rw (ints), ax, bx, ay, by, regular (int)].
+__global__ void k_tile_regular(const int32_t* __restrict__ tcnt, const float* __restrict__ px,
+                               const float* __restrict__ py, const float4* __restrict__ N4, float* __restrict__ TR) {
+    const int64_t t = blockIdx.x;
+    const int i = threadIdx.x;
+    __shared__ int ok;
+    __shared__ TileGeo g;
+    const int cnt = tcnt[t];
+    const size_t b = (size_t)t * kTile;
+    if (i == 0) {
+        ok = 0;
+        g.col0 = g.row0 = g.sh = 0;
+        g.ax = g.bx = g.ay = g.by = 0.f;
+        const float x0 = px[b], y0 = py[b];
+        int rw = 0;
+        while (rw < cnt && rw <= 64 && py[b + rw] == y0) ++rw;
+        const bool centre = x0 >= 0.f && y0 >= 0.f && x0 - floorf(x0) == 0.5f && y0 - floorf(y0) == 0.5f;
+        if (centre && rw >= 2 && rw <= 64 && (rw & (rw - 1)) == 0) {
+            int sh = 0;
+            while ((1 << sh) < rw) ++sh;
+            const float4 n0 = N4[b], n1 = N4[b + 1];
+            const float xi0 = 2.f * (n0.y + n0.z) - 1.f, xi1 = 2.f * (n1.y + n1.z) - 1.f;
+            const float eta0 = 2.f * (n0.z + n0.w) - 1.f;
+            g.col0 = (int)x0;
+            g.row0 = (int)y0;
+            g.sh = sh;
+            g.ax = xi1 - xi0;
+            g.bx = xi0 - g.ax * (float)g.col0;
+            if (cnt > rw) {
+                const float4 nr = N4[b + rw];
+                g.ay = 2.f * (nr.z + nr.w) - 1.f - eta0;
+            }
+            g.by = eta0 - g.ay * (float)g.row0;
+            ok = 1;
+        }
+    }
+    __syncthreads();
+    if (ok && i < cnt) {
+        int col, row;
+        float4 N;
+        regular_pixel(g, i, col, row, N);
+        const float4 s4 = N4[b + i];
+        const bool same = px[b + i] == (float)col + 0.5f && py[b + i] == (float)row + 0.5f &&
+                          __float_as_int(s4.x) == __float_as_int(N.x) && __float_as_int(s4.y) == __float_as_int(N.y) &&
+                          __float_as_int(s4.z) == __float_as_int(N.z) && __float_as_int(s4.w) == __float_as_int(N.w);
+        if (!same) ok = 0;  // benign race: every writer stores 0
+    }
+    __syncthreads();
+    if (i == 0) {
+        float* r = TR + (size_t)t * kTR;
+        r[0] = __int_as_float(g.col0);
+        r[1] = __int_as_float(g.row0);
+        r[2] = __int_as_float(g.sh);
+        r[3] = g.ax;
+        r[4] = g.bx;
+        r[5] = g.ay;
+        r[6] = g.by;
+        r[7] = __int_as_float(ok);
+    }
+}
+
 // Padded layout defaults: pixel centre (0.5, 0.5), zero shape functions.
 __global__ void k_pad_init(int64_t n, float* __restrict__ px, float* __restrict__ py, float4* __restrict__ N4) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1860,6 +2004,7 @@ ProbDesc make_desc(const d2ic_cg_plan* pl, const float* def_image, int32_t H, in
     d.D[1] = pl->D[1];
     d.TP = pl->TP;
     d.TV = pl->TV;
+    d.TR = pl->TR;
     d.NCP = pl->NCP;
     d.img = def_image;
     d.hist = hist;
@@ -2042,6 +2187,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PA(pl->tile_nodes, nt);
         PA(pl->TP, 8 * (size_t)nt);
         PA(pl->TV, (size_t)kTV * nt);
+        PA(pl->TR, (size_t)kTR * nt);
         PA(pl->node_ent, 4 * (size_t)nt);
         PA(pl->px, npad);
         PA(pl->py, npad);
@@ -2066,6 +2212,10 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PH(hipGetLastError());
         k_tv_static<<<ceil_div(nt, tb), tb, 0, st>>>(pl->tile_cnt, nt, pl->px, pl->py, pl->TV);
         PH(hipGetLastError());
+        if (nt > 0) {
+            k_tile_regular<<<nt, kTile, 0, st>>>(pl->tile_cnt, pl->px, pl->py, pl->N4, pl->TR);
+            PH(hipGetLastError());
+        }
 
         // node -> (tile, local node) gather table
         int64_t ne = 4 * (int64_t)nt;

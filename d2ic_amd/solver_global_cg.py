@@ -15,6 +15,7 @@ available) or "linear"/"bilinear" (the reference's in-repo bilinear sampler,
 from __future__ import annotations
 
 import ctypes
+import sys
 import warnings
 from dataclasses import dataclass
 from typing import Any
@@ -154,8 +155,6 @@ class CGPlan:
             self.handle = ctypes.c_void_p(0)
 
     def __del__(self):
-        import sys
-
         if sys.is_finalizing():  # the HIP runtime may already be torn down
             return
         try:

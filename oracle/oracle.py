@@ -34,7 +34,8 @@ What pins each function (tests named in brackets):
   vs. central finite differences in FP64 [test_oracle_objective].
 * ``cg_solve`` -- recovers a known rigid translation on synthetic speckle,
   n_iters/history semantics (stop before update, NaN padding, alpha = 1 on
-  the first accepted trial) and FP32 vs FP64 agreement [test_oracle_cg].
+  the first accepted trial) and FP32 vs FP64 agreement [test_oracle_objective];
+  the NLCG loop past iteration 0 against a numpy restatement [test_oracle_nlcg].
 * ``zncc`` -- the reference's known-shift fixture
   (``tests/test_solver_translation_zncc.py:8-33``) and a numpy brute force
   [test_oracle_zncc].
