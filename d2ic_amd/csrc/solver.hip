@@ -7,8 +7,9 @@
 // Layout (built once per PixelAssets by d2ic_cg_plan_create):
 //   * pixels grouped by element and cut into tiles of <= 256 pixels (4 chunks
 //     of 64); per pixel structure-of-arrays x, y, N0..N3 (float4) and the
-//     pre-sampled f(x_p); per tile its pixel count, 4 node ids and static
-//     pixel bounding box;
+//     pre-sampled f(x_p); per tile its pixel count, 4 node ids, static
+//     pixel bounding box and regular-geometry record (a regular tile's
+//     pixel positions and shape values are computed, so it reads only f);
 //   * a node -> (tile, local node) gather table, so the image part of the
 //     gradient is assembled deterministically (no float atomics);
 //   * the spring stencil with the combined weight w_ij * (c_i + c_j) / 2.
