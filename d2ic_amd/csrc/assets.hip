@@ -372,6 +372,10 @@ int d2ic_pixel_map(const float* nodes, int64_t n_nodes, const int32_t* elements,
     D2IC_CHECK_ARG(0 <= i0 && i0 <= i1 && i1 <= H && 0 <= j0 && j0 <= j1 && j1 <= W, "bbox outside image");
     int64_t nj = j1 - j0, ncand = (int64_t)(i1 - i0) * nj;
     D2IC_CHECK_ARG(ncand < (int64_t)INT32_MAX, "too many candidate pixels");
+    D2IC_CHECK_ARG(n_nodes >= 0 && n_elems >= 0, "sizes must be non-negative");
+    D2IC_CHECK_ARG(ncand == 0 || cand_elt != nullptr, "cand_elt must not be NULL");
+    D2IC_CHECK_ARG(ncand == 0 || n_elems == 0 || (all_set(nodes, elements) && n_nodes > 0),
+                   "nodes and elements must not be NULL");
     *n_kept = 0;
     if (ncand == 0) return D2IC_OK;
     if (n_elems == 0) {
@@ -445,7 +449,10 @@ int d2ic_pixel_fill(const int32_t* cand_elt, int32_t H, int32_t W, int32_t i0, i
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(0 <= i0 && i0 <= i1 && i1 <= H && 0 <= j0 && j0 <= j1 && j1 <= W, "bbox outside image");
     int64_t nj = j1 - j0, ncand = (int64_t)(i1 - i0) * nj;
+    D2IC_CHECK_ARG(n_pixels >= 0, "n_pixels must be >= 0");
     if (n_pixels == 0 || ncand == 0) return D2IC_OK;
+    D2IC_CHECK_ARG(all_set(cand_elt, nodes, elements, pixel_coords, pixel_nodes, pixel_shapeN, pixel_elements, roi_flat),
+                   "arrays must not be NULL");
     int64_t nblk = ceil_div(ncand, kCompactBlock);
     DevBuf bc, bo, src;
     D2IC_HIP(bc.alloc(sizeof(int32_t) * nblk, st));
@@ -480,6 +487,9 @@ int d2ic_node_pixel_degree(const int32_t* pixel_nodes, int64_t n_pixels, int64_t
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(max_degree != nullptr, "max_degree must not be NULL");
     D2IC_CHECK_ARG(4 * n_pixels < (int64_t)INT32_MAX, "too many pixels");
+    D2IC_CHECK_ARG(n_pixels >= 0 && n_nodes >= 0, "sizes must be non-negative");
+    D2IC_CHECK_ARG(n_nodes == 0 || degree != nullptr, "degree must not be NULL");
+    D2IC_CHECK_ARG(n_nodes == 0 || n_pixels == 0 || pixel_nodes != nullptr, "pixel_nodes must not be NULL");
     *max_degree = 0;
     if (n_nodes == 0) return D2IC_OK;
     D2IC_HIP(hipMemsetAsync(degree, 0, sizeof(int32_t) * n_nodes, st));
@@ -501,9 +511,11 @@ int d2ic_node_pixel_degree(const int32_t* pixel_nodes, int64_t n_pixels, int64_t
 int d2ic_node_pixel_table(const int32_t* pixel_nodes, const float* pixel_shapeN, int64_t n_pixels, int64_t n_nodes,
                           int32_t max_degree, int32_t* node_pixel_index, float* node_N_weight, void* stream) {
     hipStream_t st = (hipStream_t)stream;
-    D2IC_CHECK_ARG(max_degree >= 0, "max_degree must be >= 0");
+    D2IC_CHECK_ARG(max_degree >= 0 && n_pixels >= 0 && n_nodes >= 0, "sizes must be non-negative");
     int64_t tab = n_nodes * (int64_t)max_degree;
     if (tab == 0) return D2IC_OK;
+    D2IC_CHECK_ARG(all_set(node_pixel_index, node_N_weight), "output tables must not be NULL");
+    D2IC_CHECK_ARG(n_pixels == 0 || all_set(pixel_nodes, pixel_shapeN), "pixel arrays must not be NULL");
     D2IC_HIP(hipMemsetAsync(node_pixel_index, 0xFF, sizeof(int32_t) * tab, st));
     D2IC_HIP(hipMemsetAsync(node_N_weight, 0, sizeof(float) * tab, st));
     if (n_pixels == 0) return D2IC_OK;

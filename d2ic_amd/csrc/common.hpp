@@ -34,6 +34,13 @@ const char* last_error();
         }                                                                              \
     } while (0)
 
+// true when no pointer is NULL (argument checks of the entry points)
+inline bool all_set() { return true; }
+template <class P, class... R>
+inline bool all_set(const P* p, const R*... r) {
+    return p != nullptr && all_set(r...);
+}
+
 #define D2IC_LAUNCH_CHECK()                                                            \
     do {                                                                               \
         hipError_t _e = hipGetLastError();                                             \

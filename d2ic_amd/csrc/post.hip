@@ -512,7 +512,9 @@ int d2ic_zncc_match(const float* centers, int64_t n_centers, const float* ref, i
     D2IC_CHECK_ARG(win > 0 && win % 2 == 1, "win must be odd and positive");
     D2IC_CHECK_ARG(search >= 0, "search must be >= 0");
     D2IC_CHECK_ARG(Hr > 0 && Wr > 0 && Hd > 0 && Wd > 0, "images must be non-empty");
+    D2IC_CHECK_ARG(n_centers >= 0, "n_centers must be >= 0");
     if (n_centers == 0) return D2IC_OK;
+    D2IC_CHECK_ARG(all_set(centers, ref, def, u_out, score_out), "arrays must not be NULL");
     const int P = win + 2 * search;
     size_t lds = sizeof(float) * ((size_t)win * win + (size_t)P * P);
     D2IC_CHECK_ARG(lds <= 60 * 1024, "win + 2*search too large for the LDS-staged search (%d)", P);
@@ -525,7 +527,10 @@ int d2ic_zncc_match(const float* centers, int64_t n_centers, const float* ref, i
 int d2ic_idw(const float* nodes, int64_t n_nodes, const float* centers, const float* disp, int64_t n_centers,
              float* u_out, void* stream) {
     hipStream_t st = (hipStream_t)stream;
+    D2IC_CHECK_ARG(n_nodes >= 0 && n_centers >= 0, "sizes must be non-negative");
     if (n_nodes == 0) return D2IC_OK;
+    D2IC_CHECK_ARG(all_set(nodes, u_out), "nodes and u_out must not be NULL");
+    D2IC_CHECK_ARG(n_centers == 0 || all_set(centers, disp), "centers and disp must not be NULL");
     if (n_centers == 0) {
         D2IC_HIP(hipMemsetAsync(u_out, 0, sizeof(float) * 2 * n_nodes, st));
         return D2IC_OK;
@@ -540,7 +545,10 @@ int d2ic_strain(const float* u, const float* nodes, int64_t n_nodes, const int32
                 float* E_out, float* voigt, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(voigt != nullptr, "voigt must not be NULL");
+    D2IC_CHECK_ARG(n_nodes >= 0 && max_degree >= 0, "sizes must be non-negative");
     if (n_nodes == 0) return D2IC_OK;
+    D2IC_CHECK_ARG(all_set(u, nodes, nb_degree, F_out, E_out), "arrays must not be NULL");
+    D2IC_CHECK_ARG(max_degree == 0 || nb_index != nullptr, "nb_index must not be NULL");
     k_strain<<<ceil_div(n_nodes, 256), 256, 0, st>>>(u, nodes, n_nodes, nb_index, nb_degree, max_degree,
                                                      gauge_length, eps, F_out, E_out, voigt);
     D2IC_LAUNCH_CHECK();
@@ -553,6 +561,9 @@ int d2ic_discrepancy_map(const float* ref_image, const float* def_image, int32_t
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(H > 0 && W > 0, "image shape must be positive");
     D2IC_CHECK_ARG(n_pixels >= 0, "n_pixels must be >= 0");
+    D2IC_CHECK_ARG(map_out != nullptr, "map_out must not be NULL");
+    D2IC_CHECK_ARG(n_pixels == 0 || all_set(ref_image, def_image, u, pixel_coords, pixel_nodes, pixel_shapeN, roi_flat),
+                   "arrays must not be NULL");
     int64_t hw = (int64_t)H * W;
     if (n_pixels == 0) {
         k_nan_fill<<<ceil_div(hw, 256), 256, 0, st>>>(map_out, hw);
@@ -591,6 +602,8 @@ int d2ic_discrepancy_map(const float* ref_image, const float* def_image, int32_t
 int d2ic_render_speckle(const double* spots, int64_t n_spots, int32_t H, int32_t W, double* out, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(H > 0 && W > 0, "image shape must be positive");
+    D2IC_CHECK_ARG(out != nullptr, "out must not be NULL");
+    D2IC_CHECK_ARG(n_spots >= 0 && (n_spots == 0 || spots != nullptr), "spots must not be NULL");
     const int nbx = (W + kSpotBin - 1) / kSpotBin, nby = (H + kSpotBin - 1) / kSpotBin, nbins = nbx * nby;
     Tmp keys, vals, keys_s, order, start, tmp;
     const int64_t n = n_spots;

@@ -2038,6 +2038,11 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
     D2IC_CHECK_ARG(n_pixels >= 0 && n_nodes >= 0 && nb_max_degree >= 0, "sizes must be non-negative");
     D2IC_CHECK_ARG(n_nodes < (1 << 21), "at most 2^21 - 1 nodes are supported");
     D2IC_CHECK_ARG(4 * n_pixels < (int64_t)INT32_MAX, "too many pixels");
+    D2IC_CHECK_ARG(n_pixels == 0 || all_set(pixel_coords, pixel_nodes, pixel_shapeN, pixel_elements),
+                   "pixel arrays must not be NULL");
+    D2IC_CHECK_ARG(n_nodes == 0 || nb_degree != nullptr, "node_neighbor_degree must not be NULL");
+    D2IC_CHECK_ARG(n_nodes * (int64_t)nb_max_degree == 0 || all_set(nb_index, nb_weight, reg_weight),
+                   "neighbour tables must not be NULL");
     *plan_out = nullptr;
     d2ic_cg_plan* pl = new d2ic_cg_plan();
     auto fail = [&](int code) {

@@ -408,6 +408,18 @@ int FN(orc_node_neighbor_dense)(const i32* elems, i64 Ne, const REAL* nodes, i64
 /* ------------------------------------------------------------------------ */
 static inline i64 clampi(i64 v, i64 lo, i64 hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+/* Integer cell of a floored coordinate.  Coordinates far outside the image
+ * (or NaN, from a diverged iterate) would overflow the conversion; every tap
+ * of such a sample is clamped to the image edge anyway, so the cell is
+ * clamped to +-2^40 first (NaN: cell 0; its weights are NaN) and finite
+ * in-range results are unchanged. */
+static inline i64 cell_index(REAL f) {
+    if (!(f == f)) return 0;
+    if (f < (REAL)-1099511627776.0) return -1099511627776LL;
+    if (f > (REAL)1099511627776.0) return 1099511627776LL;
+    return (i64)f;
+}
+
 /* _bilinear_sample on the transposed image (solver_global_cg.py:481-509).
  * The image is (H, W) row-major.  The clip bounds come from the transposed
  * shape (x index clipped by H-1, y index by W-1) and the gather then clamps
@@ -416,7 +428,7 @@ static inline i64 clampi(i64 v, i64 lo, i64 hi) { return v < lo ? lo : (v > hi ?
 static void sample_linear(const REAL* img, i64 H, i64 W, REAL x, REAL y, REAL* v, REAL* gx, REAL* gy) {
     REAL xs = x - (REAL)0.5, ys = y - (REAL)0.5;
     REAL x0f = (REAL)floor((double)xs), y0f = (REAL)floor((double)ys);
-    i64 x0 = (i64)x0f, y0 = (i64)y0f;
+    i64 x0 = cell_index(x0f), y0 = cell_index(y0f);
     REAL fx = xs - x0f, fy = ys - y0f;
     i64 xa = clampi(clampi(x0, 0, H - 1), 0, W - 1), xb = clampi(clampi(x0 + 1, 0, H - 1), 0, W - 1);
     i64 ya = clampi(clampi(y0, 0, W - 1), 0, H - 1), yb = clampi(clampi(y0 + 1, 0, W - 1), 0, H - 1);
@@ -451,7 +463,7 @@ static void keys_weights(REAL t, REAL w[4], REAL dw[4]) {
 static void sample_cubic(const REAL* img, i64 H, i64 W, REAL x, REAL y, REAL* v, REAL* gx, REAL* gy) {
     REAL xs = x - (REAL)0.5, ys = y - (REAL)0.5;
     REAL x0f = (REAL)floor((double)xs), y0f = (REAL)floor((double)ys);
-    i64 x0 = (i64)x0f, y0 = (i64)y0f;
+    i64 x0 = cell_index(x0f), y0 = cell_index(y0f);
     REAL tx = xs - x0f, ty = ys - y0f;
     REAL wx[4], dwx[4], wy[4], dwy[4];
     keys_weights(tx, wx, dwx);

@@ -68,6 +68,10 @@ import numpy as np
 
 _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "_build" / "liboracle.so"
+if os.environ.get("D2IC_ORACLE_SANITIZED") == "1":
+    # AddressSanitizer + UBSan build of the same source (``make sanitized``);
+    # the process must have loaded libasan first
+    _LIB_PATH = _HERE / "_build" / "liboracle_san.so"
 _lib = None
 
 _i64 = ctypes.c_int64
@@ -80,7 +84,8 @@ _dbl = ctypes.c_double
 def build(force: bool = False) -> Path:
     """Compile the oracle with its Makefile (gcc)."""
     if force or not _LIB_PATH.exists():
-        subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+        target = "sanitized" if _LIB_PATH.name == "liboracle_san.so" else "all"
+        subprocess.run(["make", "-s", "-C", str(_HERE), target], check=True)
     return _LIB_PATH
 
 
