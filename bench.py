@@ -164,9 +164,62 @@ def batch_leg(seed=7, n=64, max_iters=200, tol=1e-3, reg=0.1, accuracy_samples=3
                init="ZNCC (win 41, search 4) at <=4096 element centres + IDW, shared by both paths")
     log(f"[batch] batched {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, speedup {t_plain / t_batch:.2f}, "
         f"bitwise equal {same}; accuracy {acc}")
+    res["graph_replay"] = _graph_replay(probs, itp, reg, max_iters, tol)
     for p in probs:
         p["plan"].close()
     return res
+
+
+def _graph_replay(probs, it_plain, reg, max_iters, tol):
+    """The batch's 512^2 problems solved one by one, eagerly (d2ic_cg_solve)
+    and by replaying one captured solve graph per problem (SolveGraph), from
+    the same initial guess: the per-solve launch cost a graph removes."""
+    import ctypes
+
+    import torch
+
+    from d2ic_amd import _lib
+    from d2ic_amd.solver_global_cg import SolveGraph
+
+    idx = [i for i, p in enumerate(probs) if p["H"] == 512]
+    if not idx:
+        return None
+    graphs = [SolveGraph([probs[i]["plan"]], [probs[i]["img"]], reg_strength=reg, max_iter=max_iters, tol=tol,
+                         u=[probs[i]["u0"]]) for i in idx]
+    st = _lib.stream()
+
+    def eager():
+        us = []
+        for i in idx:
+            p = probs[i]
+            u = p["u0"].clone()
+            cnt = torch.zeros(2, dtype=torch.int32, device="cuda")
+            _lib.call("d2ic_cg_solve", p["plan"].handle, _lib.ptr(p["img"]), p["H"], p["H"], _lib.ptr(u), reg,
+                      max_iters, tol, None, _lib.ptr(cnt), ctypes.c_void_p(cnt.data_ptr() + 4), st)
+            us.append(u)
+        torch.cuda.synchronize()
+        return us
+
+    def replay():
+        for i, g in zip(idx, graphs):
+            g.u[0].copy_(probs[i]["u0"])
+            g.replay()
+        torch.cuda.synchronize()
+
+    eager()
+    replay()
+    t0 = time.perf_counter()
+    ue = eager()
+    t_eager = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    replay()
+    t_graph = time.perf_counter() - t0
+    same = all(torch.equal(g.u[0], u) for g, u in zip(graphs, ue)) and \
+        all(g.results()[0][1] == it_plain[i] for g, i in zip(graphs, idx))
+    log(f"[batch] graph replay of {len(idx)} 512^2 solves: {t_graph * 1e3:.1f} ms vs eager {t_eager * 1e3:.1f} ms, "
+        f"bitwise equal {same}")
+    return dict(problems=len(idx), eager_ms=t_eager * 1e3, graph_ms=t_graph * 1e3, bitwise_equal=bool(same),
+                note="includes copying each warm start into the captured buffer before its replay")
 
 
 def serving_leg(seed=11, n_streams=16, rate=200.0, seconds=1.0, max_iters=200, tol=1e-3, reg=0.1):

@@ -2403,10 +2403,16 @@ int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, Pro
     // tools that cannot follow cooperative launches).  The grid never exceeds
     // the co-resident capacity; if blocks were not all resident the bounded
     // barrier wait would end the solve with n_iters = -1 instead of hanging.
-    static const bool plain_launch = [] {
+    static const bool plain_env = [] {
         const char* e = getenv("D2IC_PLAIN_LAUNCH");
         return e && e[0] == '1';
     }();
+    // A stream being captured into a HIP graph records an ordinary launch of
+    // the same grid (the replay then relies on the grid being co-resident, as
+    // above: the graph must not run beside other kernels that hold CUs).
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    D2IC_HIP(hipStreamIsCapturing(st, &cap));
+    const bool plain_launch = plain_env || cap != hipStreamCaptureStatusNone;
     if (plain_launch) {
         if (kern == (const void*)k_cg_persistent<false, true, 0>)
             k_cg_persistent<false, true, 0><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);

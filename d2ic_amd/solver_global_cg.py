@@ -163,6 +163,85 @@ class CGPlan:
             pass
 
 
+class SolveGraph:
+    """A whole batched solve captured once into a HIP graph and replayed per
+    frame (no reference counterpart: the reference re-enters its jitted
+    ``_cg_solve`` for every frame).
+
+    The graph holds the launch sequence of ``d2ic_cg_solve_batch_noalloc`` --
+    descriptor upload, workspace clear, per-problem init, the persistent NLCG
+    kernel that runs every iteration, finalize -- for fixed buffers: the
+    caller writes the next frame into ``images[p]`` and the warm start into
+    ``u[p]`` (in place), then calls :meth:`replay`; ``n_iters``,
+    ``pass_count`` and ``hist`` are refreshed by every replay.  Up to 8
+    problems (the allocation-free entry point's scratch array).  Every solve
+    on a GPU that uses that array (replays of any graph, direct
+    ``d2ic_cg_solve_batch_noalloc`` calls) must be serialised, and a replay
+    must not run beside other kernels that hold CUs: the captured persistent
+    kernel is an ordinary launch whose grid must be co-resident.
+    """
+
+    def __init__(self, plans, images, reg_strength=0.1, max_iter: int = 200, tol: float = 1e-3, u=None,
+                 with_history: bool = False):
+        import ctypes as _ct
+
+        t = _lib.torch()
+        plans = list(plans)
+        P = len(plans)
+        if not 1 <= P <= 8:
+            raise ValueError("a solve graph holds 1 to 8 problems")
+        if len(images) != P:
+            raise ValueError("one image per plan")
+        for pl in plans:
+            if pl._ref_key is None:
+                raise RuntimeError("set_reference() must be called on every plan before capture")
+        self.plans = plans
+        self.images = [_lib.to_dev(g, t.float32).contiguous().clone() for g in images]
+        for g in self.images:
+            if g.dim() != 2:
+                raise ValueError("images must be 2-D")
+        self.u = [t.zeros((pl.n_nodes, 2), dtype=t.float32, device=_lib.device()) for pl in plans]
+        if u is not None:
+            for dst, src in zip(self.u, u):
+                dst.copy_(_lib.to_dev(src, t.float32, dst.shape))
+        self.hist = [t.empty((max(int(max_iter), 1), 3), dtype=t.float32, device=_lib.device())
+                     if with_history else None for _ in plans]
+        self.counters = [t.zeros(2, dtype=t.int32, device=_lib.device()) for _ in plans]
+        self.max_iter = int(max_iter)
+        vp = _ct.c_void_p
+        arr = lambda xs: (vp * P)(*[vp(x) for x in xs])  # noqa: E731
+        self._args = (P, arr([pl.handle.value for pl in plans]), arr([g.data_ptr() for g in self.images]),
+                      (_ct.c_int32 * P)(*[g.shape[0] for g in self.images]),
+                      (_ct.c_int32 * P)(*[g.shape[1] for g in self.images]),
+                      arr([x.data_ptr() for x in self.u]),
+                      (_ct.c_float * P)(*([float(reg_strength)] * P) if np.isscalar(reg_strength)
+                                        else [float(v) for v in reg_strength]),
+                      (_ct.c_int32 * P)(*([self.max_iter] * P)), (_ct.c_float * P)(*([float(tol)] * P)),
+                      arr([_lib.ptr(h) for h in self.hist]), arr([c.data_ptr() for c in self.counters]),
+                      arr([c.data_ptr() + 4 for c in self.counters]))
+        # one eager call validates the arguments and resolves the scratch
+        # array's device address before capture; the warm start is restored
+        u0 = [x.clone() for x in self.u]
+        _lib.call("d2ic_cg_solve_batch_noalloc", *self._args, _lib.stream())
+        for dst, src in zip(self.u, u0):
+            dst.copy_(src)
+        t.cuda.synchronize()
+        self.graph = t.cuda.CUDAGraph()
+        side = t.cuda.Stream()
+        side.wait_stream(t.cuda.current_stream())
+        with t.cuda.graph(self.graph, stream=side):
+            _lib.call("d2ic_cg_solve_batch_noalloc", *self._args, int(side.cuda_stream))
+        t.cuda.current_stream().wait_stream(side)
+
+    def replay(self):
+        """Run the captured solve on the current contents of ``images`` and ``u``."""
+        self.graph.replay()
+
+    def results(self):
+        """[(u, n_iters, pass_count)] of the last replay (synchronises)."""
+        return [(u, int(c[0]), int(c[1])) for u, c in zip(self.u, (c.cpu() for c in self.counters))]
+
+
 class GlobalCGSolver(SolverBase):
     """See module docstring.  Same constructor as the reference
     (``solver_global_cg.py:45-63``)."""
