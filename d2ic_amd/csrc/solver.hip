@@ -426,26 +426,27 @@ __device__ __forceinline__ void box_load(const ProbDesc* __restrict__ D, const T
     if (!g.staged) return;
     const int n = g.bw * g.bh, interp = kInterp >= 0 ? kInterp : D->interp, H = D->H, W = D->W;
     const D2IC_GLOBAL float* img = gld(D->img);
-    const float inv_bw = 1.0f / (float)g.bw;
+    // hardware reciprocal (about 1 ulp): (k + 0.5) / bw stays at least 1e-3
+    // away from an integer for k < 512, so the truncation gives the exact row
+    const float inv_bw = __builtin_amdgcn_rcpf((float)g.bw);
     // the bilinear sampler clips by the transposed shape: its taps are the
     // pixels themselves only inside [0, min(H, W))^2
     const int lim_w = interp == 0 ? W : min(W, H), lim_h = interp == 0 ? H : min(W, H);
     if (g.bx0 >= 0 && g.by0 >= 0 && g.bx0 + g.bw <= lim_w && g.by0 + g.bh <= lim_h) {
-        // box inside the image: no clamps; element k = j * 64 + lane at (r, c),
-        // advanced by 64 = rs * bw + cs per register; rows past the box repeat
-        // its last row (those values are never stored)
+        // box inside the image: element k = j * 64 + lane (clamped to the
+        // box's last element; lanes past it are never stored) at row
+        // r = k / bw, column k - r bw, is image element k + r (W - bw) from the
+        // box origin (24-bit multiply: full rate)
         const D2IC_GLOBAL float* base = img + (size_t)g.by0 * W + g.bx0;
-        const int rs = (int)(64.5f * inv_bw), cs = kWave - rs * g.bw;
-        int r = (int)(((float)lane + 0.5f) * inv_bw);
-        int c = lane - r * g.bw;
+        const unsigned step = (unsigned)(W - g.bw);
+        const float half_inv = 0.5f * inv_bw;
+        const int nm1 = n - 1;
 #pragma unroll
         for (int j = 0; j < kBoxRegs; ++j) {
-            if (j * kWave < n) v[j] = base[min(r, g.bh - 1) * W + c];
-            c += cs;
-            r += rs;
-            if (c >= g.bw) {
-                c -= g.bw;
-                r += 1;
+            if (j * kWave < n) {
+                const int k = min(j * kWave + lane, nm1);
+                const unsigned r = (unsigned)(int)__builtin_fmaf((float)k, inv_bw, half_inv);
+                v[j] = base[(unsigned)k + __umul24(r, step)];
             }
         }
         return;
@@ -467,11 +468,11 @@ __device__ __forceinline__ void box_store(const TileBox& g, int lane, const floa
     if (!g.staged) return;
     const int n = g.bw * g.bh;
     __builtin_amdgcn_wave_barrier();
+    // lanes past the box's last element write inside the wave's own slot
+    // (k < kBoxMax); those entries are never read
 #pragma unroll
-    for (int j = 0; j < kBoxRegs; ++j) {
-        const int k = j * kWave + lane;
-        if (j * kWave < n && k < n) box[k] = v[j];
-    }
+    for (int j = 0; j < kBoxRegs; ++j)
+        if (j * kWave < n) box[j * kWave + lane] = v[j];
     __builtin_amdgcn_wave_barrier();
 }
 
