@@ -49,8 +49,8 @@ _SIGS = {
     "d2ic_cg_plan_set_grid": [_vp, _i32],
     "d2ic_cg_kernel_ms": [_vp, ctypes.POINTER(_f32)],
     "d2ic_serve_create": [_i32, ctypes.c_double, ctypes.POINTER(_vp)],
-    "d2ic_serve_submit": [_vp, _vp, _vp, _i32, _i32, _vp, _f32, _i32, _f32, _vp, _vp, _vp, ctypes.POINTER(_i32),
-                          ctypes.POINTER(_i64)],
+    "d2ic_serve_submit": [_vp, _vp, _vp, _i32, _i32, _vp, _vp, _f32, _i32, _f32, _vp, _vp, _vp,
+                          ctypes.POINTER(_i32), ctypes.POINTER(_i64)],
     "d2ic_serve_poll": [_vp, _i32, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     "d2ic_serve_destroy": [_vp],
     "d2ic_adj_sizes": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],

@@ -94,6 +94,7 @@ struct ProbDesc {
     const int32_t* nb_idx;
     const float* nb_wp;
     float* U[2];
+    const float* U0;  // initial guess, read by the first evaluation only (U[0] itself, or a separate buffer)
     float* G[2];
     float* D[2];
     float* TP;
@@ -320,7 +321,7 @@ __device__ __forceinline__ int tile_ids(const ProbDesc* __restrict__ D, int t, i
 __device__ __forceinline__ float tile_vals(const ProbDesc* __restrict__ D, int id, int lane, int flags) {
     if (lane < 24) {
         const int which = lane >> 3;
-        const float* base = which == 0 ? D->U[(flags & F_CUR) ? 1 : 0]
+        const float* base = which == 0 ? ((flags & F_INIT) ? D->U0 : D->U[(flags & F_CUR) ? 1 : 0])
                                        : (which == 1 ? D->G[(flags & F_CUR) ? 1 : 0] : D->D[(flags & F_DCUR) ? 1 : 0]);
         return gld(base)[2 * (size_t)id + (lane & 1)];
     }
@@ -677,7 +678,7 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
     float2 un = make_float2(0.f, 0.f), go = make_float2(0.f, 0.f), dn = make_float2(0.f, 0.f);
     double sx = 0.0, sy = 0.0;
     float rx = 0.f, ry = 0.f;
-    const float* U = D->U[cur];
+    const float* U = init ? D->U0 : D->U[cur];
     const float* G = D->G[cur];
     const float* Dp = D->D[dcur];
     if (valid) {
@@ -829,6 +830,10 @@ __device__ int decide(const DecideDesc& D, ProbState& st, double sr2, bool bad, 
     return flags;
 }
 
+// Result codes of served requests (published as the iteration count).
+constexpr int kServeNonFinite = -2;  // the final objective or gradient is not finite
+constexpr int kServeBadStart = -3;   // the objective at the initial guess is not finite
+
 // Every block, right after barrier `step`: reduce the finished step from the
 // same memory in the same order, run the same state machine on its own copy
 // of the problem states, and derive the next step's plan (identical in every
@@ -891,15 +896,29 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
             ProbState st = s_st[p];
             const int prev = st.mode;
             bool finished = false;
-            int fl2;
+            int fl2, code = 0;
             if (prev == MODE_COPY) {  // the copy pass ran in this step
                 st.mode = MODE_DONE;
                 fl2 = 0;
                 finished = true;
             } else {
                 fl2 = decide(D, st, sr2, bad != 0, sg, sn, sd, sE, writer);
+                if (sh) {
+                    // served requests end early on a non-finite objective at the
+                    // initial guess (the frame or u0 holds NaN/Inf where it is
+                    // sampled), and report a non-finite final objective or
+                    // gradient; the result buffer is then not filled
+                    if (prev == MODE_INIT && !isfinite(st.J)) {
+                        st.mode = MODE_DONE;
+                        st.passes -= 1;  // decide() counted the next pass, which does not run
+                        fl2 = 0;
+                        code = kServeBadStart;
+                    } else if (prev != MODE_DONE && st.mode == MODE_DONE && !(isfinite(st.J) && isfinite(sg))) {
+                        code = kServeNonFinite;
+                    }
+                }
                 if (prev != MODE_DONE && st.mode == MODE_DONE) {
-                    if (sh && st.cur) {
+                    if (sh && st.cur && code == 0) {
                         st.mode = MODE_COPY;
                         fl2 = F_COPY;
                     } else {
@@ -921,7 +940,8 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
             }
             if (finished && writer && sh) {
                 if (D.pass_count) *D.pass_count = st.passes;
-                sh->done_iters[p] = st.k;
+                if (code) *D.n_iters = code;
+                sh->done_iters[p] = code ? code : st.k;
                 sh->done_passes[p] = st.passes;
                 __threadfence_system();
                 __hip_atomic_store(&sh->done_gen[p],
@@ -1985,7 +2005,8 @@ int occupancy_blocks(const void* kernel, int* out) {
 namespace {
 
 ProbDesc make_desc(const d2ic_cg_plan* pl, const float* def_image, int32_t H, int32_t W, float* u, float alpha_reg,
-                   int32_t max_iter, float tol, float* hist, int32_t* n_iters, int32_t* pass_count) {
+                   int32_t max_iter, float tol, float* hist, int32_t* n_iters, int32_t* pass_count,
+                   const float* u0 = nullptr) {
     ProbDesc d;
     d.tile_cnt = pl->tile_cnt;
     d.tile_nodes = pl->tile_nodes;
@@ -2000,6 +2021,7 @@ ProbDesc make_desc(const d2ic_cg_plan* pl, const float* def_image, int32_t H, in
     d.nb_wp = pl->nb_wp;
     d.U[0] = u;
     d.U[1] = pl->U1;
+    d.U0 = u0 ? u0 : u;
     d.G[0] = pl->G[0];
     d.G[1] = pl->G[1];
     d.D[0] = pl->D[0];
@@ -2601,8 +2623,19 @@ static int serve_launch(d2ic_server* srv) {
     int max_steps = INT32_MAX;
     void* args[] = {(void*)&srv->d_desc, (void*)&P, (void*)&srv->d_S, (void*)&srv->d_ws, (void*)&max_steps,
                     (void*)&srv->sh};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_cg_persistent<true>, dim3(nblocks), dim3(kBlock), args, 0,
-                                              srv->stream);
+    static const bool plain_env = [] {
+        const char* v = getenv("D2IC_PLAIN_LAUNCH");
+        return v && v[0] == '1';
+    }();
+    hipError_t e = hipSuccess;
+    if (plain_env) {
+        k_cg_persistent<true><<<nblocks, kBlock, 0, srv->stream>>>(srv->d_desc, P, srv->d_S, srv->d_ws, max_steps,
+                                                                    srv->sh);
+        e = hipGetLastError();
+    } else {
+        e = hipLaunchCooperativeKernel((const void*)k_cg_persistent<true>, dim3(nblocks), dim3(kBlock), args, 0,
+                                       srv->stream);
+    }
     if (e != hipSuccess) {
         __atomic_store_n(&srv->sh->alive, 0, __ATOMIC_RELEASE);
         set_error("served kernel launch failed: %s", hipGetErrorString(e));
@@ -2653,7 +2686,7 @@ int d2ic_serve_create(int32_t slots, double idle_ms, d2ic_server** out) {
 }
 
 int d2ic_serve_submit(d2ic_server* srv, d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W, float* u,
-                      float alpha_reg, int32_t max_iter, float tol, float* hist, int32_t* n_iters,
+                      const float* u0, float alpha_reg, int32_t max_iter, float tol, float* hist, int32_t* n_iters,
                       int32_t* pass_count, int32_t* slot_out, int64_t* ticket_out) {
     D2IC_CHECK_ARG(srv && plan && slot_out && ticket_out, "server, plan, slot_out and ticket_out must not be NULL");
     D2IC_CHECK_ARG(def_image != nullptr && H > 0 && W > 0, "deformed image must be non-empty");
@@ -2676,7 +2709,8 @@ int d2ic_serve_submit(d2ic_server* srv, d2ic_cg_plan* plan, const float* def_ima
     *slot_out = slot;
     *ticket_out = 0;
     if (slot < 0) return D2IC_OK;  // every slot busy: retry after a poll frees one
-    srv->sh->desc[slot] = make_desc(plan, def_image, H, W, u, alpha_reg, max_iter, tol, hist, n_iters, pass_count);
+    srv->sh->desc[slot] =
+        make_desc(plan, def_image, H, W, u, alpha_reg, max_iter, tol, hist, n_iters, pass_count, u0);
     const unsigned long long g = ++srv->gen;
     __atomic_store_n(&srv->sh->submit_gen[slot], g, __ATOMIC_RELEASE);
     __atomic_fetch_add(&srv->sh->doorbell, 1ull, __ATOMIC_RELEASE);

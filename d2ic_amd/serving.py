@@ -19,20 +19,24 @@ Request status codes (``Request.status``):
 ``QUEUED``             waiting in its stream's queue
 ``RUNNING``            submitted to the served kernel
 ``DONE``               solved; ``u``, ``n_iters``, ``passes`` are set
-``REJECTED_INVALID``   frame not 2-D, of another shape than the stream's
-                       reference image, or not finite (NaN/Inf): never submitted
+``REJECTED_INVALID``   frame not 2-D or of another shape than the stream's
+                       reference image, or a host frame holding NaN/Inf: never
+                       submitted; or a device frame (or warm start) whose
+                       NaN/Inf values the solve samples: found by the served
+                       kernel in its first pass, which ends the request
 ``REJECTED_QUEUE``     the stream's queue already holds ``max_queue`` frames
 ``REJECTED_STREAM``    unknown (or removed) stream key
 ``CANCELLED``          cancelled while queued, or while running (the result is
                        discarded and does not become the stream's warm start)
 ``FAILED``             the solve did not finish (a grid barrier timed out) or
-                       returned non-finite displacements; the stream keeps its
-                       previous warm start
+                       ended with a non-finite objective or gradient (reported
+                       by the kernel); the stream keeps its previous warm start
 =====================  =========================================================
 """
 from __future__ import annotations
 
 import ctypes
+import sys
 import threading
 import time
 from collections import deque
@@ -52,6 +56,9 @@ REJECTED_QUEUE = "REJECTED_QUEUE"
 REJECTED_STREAM = "REJECTED_STREAM"
 CANCELLED = "CANCELLED"
 FAILED = "FAILED"
+# iteration counts the served kernel reports for failed requests
+SERVE_NON_FINITE = -2
+SERVE_BAD_START = -3
 FINAL = frozenset({DONE, REJECTED_INVALID, REJECTED_QUEUE, REJECTED_STREAM, CANCELLED, FAILED})
 
 
@@ -105,28 +112,38 @@ class ServeSession:
         self._lock = threading.Lock()
 
     def submit(self, plan: CGPlan, def_image, u, reg_strength: float, max_iter: int, tol: float,
-               save_history: bool = False) -> Ticket | None:
-        """Queue one solve; ``u`` ((Nn,2) f32) holds u0, and the returned
-        ticket's ``u`` receives the solution (``u`` itself when it is already a
-        contiguous f32 device tensor of that shape).  ``def_image`` is converted
-        to a contiguous f32 device tensor.  Returns None when every slot is busy."""
+               save_history: bool = False, u0=None, counters=None) -> Ticket | None:
+        """Queue one solve; the returned ticket's ``u`` receives the solution
+        (``u`` itself when it is already a contiguous f32 device tensor of shape
+        (Nn, 2)).  The initial guess is ``u0`` when given (only read), else
+        ``u``'s contents.  ``def_image`` is converted to a contiguous f32 device
+        tensor.  ``counters`` (2 int32 on the device, reusable) receives
+        n_iters and the pass count.  Returns None when every slot is busy.
+
+        Device inputs that are already contiguous f32 tensors start no GPU
+        work here: while the served kernel holds the GPU, other kernels may
+        wait for it to leave, so the request path avoids them."""
         t = _lib.torch()
         g, ud = prepare_inputs(plan.n_nodes, def_image, u, _lib.device())
+        u0d = None
+        if u0 is not None:
+            _, u0d = prepare_inputs(plan.n_nodes, g, u0, _lib.device())
         hist = None
         if save_history:
             hist = t.full((max(int(max_iter), 0), 3), float("nan"), dtype=t.float32, device=_lib.device())
-        counters = t.full((2,), -1, dtype=t.int32, device=_lib.device())
+        if counters is None:
+            counters = t.empty((2,), dtype=t.int32, device=_lib.device())
         t.cuda.current_stream().synchronize()  # inputs must be on the device before the kernel reads them
         slot = ctypes.c_int32(-1)
         ticket = ctypes.c_int64(0)
         with self._lock:
             _lib.call("d2ic_serve_submit", self.handle, plan.handle, _lib.ptr(g), int(g.shape[0]),
-                      int(g.shape[1]), _lib.ptr(ud), float(reg_strength), int(max_iter), float(tol),
+                      int(g.shape[1]), _lib.ptr(ud), _lib.ptr(u0d), float(reg_strength), int(max_iter), float(tol),
                       _lib.ptr(hist), _lib.ptr(counters), ctypes.c_void_p(counters.data_ptr() + 4),
                       ctypes.byref(slot), ctypes.byref(ticket))
         if slot.value < 0:
             return None
-        return Ticket(slot.value, ticket.value, ud, hist, counters, _keep=(g, plan))
+        return Ticket(slot.value, ticket.value, ud, hist, counters, _keep=(g, plan, u0d))
 
     def poll(self, tk: Ticket) -> bool:
         done = ctypes.c_int32(0)
@@ -157,8 +174,6 @@ class ServeSession:
             self.handle = ctypes.c_void_p(0)
 
     def __del__(self):
-        import sys
-
         if sys.is_finalizing():
             return
         try:
@@ -177,6 +192,7 @@ class StreamState:
     max_iter: int
     tol: float
     u_prev: Any = None
+    counters: Any = None  # n_iters and pass count of the request in flight (device)
     bytes_resident: int = 0
     queue: deque = field(default_factory=deque)
     inflight: Any = None
@@ -204,6 +220,11 @@ def _all_finite(x) -> bool:
     if isinstance(x, t.Tensor):
         return bool(t.isfinite(x).all().item())
     return bool(np.isfinite(np.asarray(x)).all())
+
+
+def _on_device(x) -> bool:
+    t = _lib.torch()
+    return isinstance(x, t.Tensor) and x.device.type != "cpu"
 
 
 class StreamServer:
@@ -257,6 +278,7 @@ class StreamServer:
             st.u_prev = _to(u0, self.device).reshape(plan.n_nodes, 2).clone()
         else:
             st.u_prev = t.zeros((plan.n_nodes, 2), dtype=t.float32, device=self.device)
+        st.counters = t.empty((2,), dtype=t.int32, device=self.device)
         st.bytes_resident = need
         self.used += need
         self.streams[key] = st
@@ -307,7 +329,10 @@ class StreamServer:
         if len(shape) != 2 or shape != st.image_shape:
             return self._finish(req, REJECTED_INVALID,
                                 f"frame shape {shape} differs from the reference image {st.image_shape}")
-        if self.check_finite and not _all_finite(frame):
+        # host frames are checked here; a device frame is checked by the served
+        # kernel's first pass (a check here would start GPU work, which can wait
+        # for the served kernel to leave the GPU)
+        if self.check_finite and not _on_device(frame) and not _all_finite(frame):
             return self._finish(req, REJECTED_INVALID, "frame holds NaN or Inf values")
         if len(st.queue) >= self.max_queue:
             return self._finish(req, REJECTED_QUEUE, f"stream queue full ({self.max_queue} frames)")
@@ -339,11 +364,13 @@ class StreamServer:
         req.n_iters, req.passes = tk.n_iters, tk.passes
         if req._cancel:
             self._finish(req, CANCELLED, "cancelled while running")
+        elif tk.n_iters == SERVE_BAD_START:
+            self._finish(req, REJECTED_INVALID,
+                         "the frame or the warm start holds NaN or Inf values where the solve samples it")
+        elif tk.n_iters == SERVE_NON_FINITE:
+            self._finish(req, FAILED, "non-finite displacements (the final objective or gradient is not finite)")
         elif tk.n_iters is None or tk.n_iters < 0:
             self._finish(req, FAILED, "the solve did not finish (a grid barrier timed out)")
-        elif not _all_finite(tk.u):
-            req.u = tk.u
-            self._finish(req, FAILED, "non-finite displacements")
         else:
             req.u = tk.u
             st.u_prev = tk.u
@@ -358,8 +385,11 @@ class StreamServer:
             n += self._collect(st)
             if st.inflight is None and st.queue and not st.closing:
                 req = st.queue[0]
-                u = st.u_prev.clone()
-                tk = self.session.submit(st.plan, req.frame, u, st.reg_strength, st.max_iter, st.tol)
+                # the solve reads the warm start from the previous result and
+                # writes a fresh buffer: no device work on the request path
+                u = _lib.torch().empty_like(st.u_prev)
+                tk = self.session.submit(st.plan, req.frame, u, st.reg_strength, st.max_iter, st.tol,
+                                         u0=st.u_prev, counters=st.counters)
                 if tk is not None:
                     st.queue.popleft()
                     req.status = RUNNING

@@ -155,14 +155,21 @@ typedef struct d2ic_server d2ic_server;
  * idle_ms without work (it is relaunched by the next submit or poll). */
 int d2ic_serve_create(int32_t slots, double idle_ms, d2ic_server** server);
 
-/* Queue one solve (same arguments as d2ic_cg_solve).  Inputs must be ready on
- * the device (synchronise their producer first); hist, if given, must already
- * hold NaN.  The request joins the running kernel at its next grid barrier.
- * *slot = -1 (and *ticket = 0) when every slot is busy: poll, then retry.
- * A plan may have one request in flight at a time.  Thread-safe. */
+/* Queue one solve (same arguments as d2ic_cg_solve, plus u0).  u receives
+ * the solution; u0 (NULL: u itself) holds the initial guess and is only read,
+ * so a stream can start frame k+1 from frame k's result buffer without a
+ * device copy.  Inputs must be ready on the device (synchronise their producer
+ * first); hist, if given, must already hold NaN.  The request joins the
+ * running kernel at its next grid barrier.  *slot = -1 (and *ticket = 0) when
+ * every slot is busy: poll, then retry.  A plan may have one request in flight
+ * at a time.  Thread-safe.  The request's iteration count (n_iters and the
+ * poll's copy) is -2 when the final objective or gradient is not finite and
+ * -3 when the objective at u0 is not finite (a NaN/Inf in the frame or in u0
+ * where the solve samples it; the request then ends after that first pass);
+ * u is not written in either case. */
 int d2ic_serve_submit(d2ic_server* server, d2ic_cg_plan* plan, const float* def_image, int32_t H, int32_t W,
-                      float* u, float alpha_reg, int32_t max_iter, float tol, float* hist, int32_t* n_iters,
-                      int32_t* pass_count, int32_t* slot, int64_t* ticket);
+                      float* u, const float* u0, float alpha_reg, int32_t max_iter, float tol, float* hist,
+                      int32_t* n_iters, int32_t* pass_count, int32_t* slot, int64_t* ticket);
 
 /* Non-blocking completion check of (slot, ticket).  *done = 1 once u, hist,
  * n_iters and pass_count hold the result; the slot is then free.  n_iters

@@ -131,3 +131,50 @@ def test_stream_server_matches_sequential(gpu):
             plan.close()
     finally:
         srv.close()
+
+
+def test_stream_server_device_frame_failures(gpu):
+    """A device frame with NaN inside the ROI is found by the served kernel's
+    first pass: the request ends REJECTED_INVALID after one pass, its result
+    buffer is not used, and the stream's next frame starts from the previous
+    warm start and equals the plain solve bitwise.  The previous results stay
+    intact (the kernel only reads the warm start)."""
+    from d2ic_amd import _lib, speckle
+    from d2ic_amd.mask2mesh import mask_to_mesh
+    from d2ic_amd.pixel_assets import build_pixel_assets
+    from d2ic_amd.serving import DONE, REJECTED_INVALID, StreamServer
+    from d2ic_amd.solver_global_cg import CGPlan
+
+    t = _lib.torch()
+    print("[case] stream server: NaN device frame", flush=True)
+    H, e = 112, 16
+    m = speckle.rect_mask(H, H, H - 32, H - 32)
+    mesh = mask_to_mesh(m, e)
+    fns = [speckle.rigid(0.2 * (f + 1), -0.15 * (f + 1)) for f in range(3)]
+    ref, frames = speckle.speckle_sequence_device(H, H, 21, fns, noise_sigma=1.0)
+    pa = build_pixel_assets(mesh, ref, 1.0, roi_mask=m)
+    bad = frames[1].clone()
+    bad[H // 2, H // 2] = float("nan")
+    srv = StreamServer(slots=2, idle_ms=2.0)
+    try:
+        assert srv.add_stream("s", pa, ref, max_iter=40, tol=1e-4)
+        r0 = srv.submit("s", frames[0])
+        r1 = srv.submit("s", bad)
+        r2 = srv.submit("s", frames[2])
+        srv.drain()
+        assert [r.status for r in (r0, r1, r2)] == [DONE, REJECTED_INVALID, DONE], [r.reason for r in (r0, r1, r2)]
+        assert r1.passes == 1 and r1.u is None and "NaN" in r1.reason
+        u0_copy = r0.u.clone()
+        plan = CGPlan(pa)
+        plan.set_reference(ref, "cubic")
+        u = t.zeros((plan.n_nodes, 2), dtype=t.float32, device=_lib.device())
+        for f, r in ((0, r0), (2, r2)):
+            cnt = t.zeros(2, dtype=t.int32, device=_lib.device())
+            _lib.call("d2ic_cg_solve", plan.handle, _lib.ptr(frames[f]), H, H, _lib.ptr(u), 0.1, 40, 1e-4, None,
+                      _lib.ptr(cnt), None, _lib.stream())
+            assert int(cnt[0].item()) == r.n_iters
+            assert t.equal(u, r.u), f
+        assert t.equal(r0.u, u0_copy)  # frame 0's result was only read by the later solves
+        plan.close()
+    finally:
+        srv.close()

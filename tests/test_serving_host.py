@@ -47,14 +47,15 @@ class FakeSession:
         self.submitted = []
         self.events = events if events is not None else []
 
-    def submit(self, plan, frame, u, reg, max_iter, tol):
+    def submit(self, plan, frame, u, reg, max_iter, tol, u0=None, counters=None):
         if len(self.running) >= self.slots:
             return None
         first = float(np.asarray(frame).reshape(-1)[0])
-        outcome = "timeout" if first == -1.0 else ("nan" if first == -2.0 else "ok")
-        tk = FakeTicket(u.clone(), self.latency, outcome)
+        outcome = {-1.0: "timeout", -2.0: "nan", -3.0: "bad_start"}.get(first, "ok")
+        start = (u0 if u0 is not None else u).clone()
+        tk = FakeTicket(start, self.latency, outcome)
         self.running.append(tk)
-        self.submitted.append((plan, frame, u.clone()))
+        self.submitted.append((plan, frame, start))
         return tk
 
     def poll(self, tk):
@@ -62,10 +63,13 @@ class FakeSession:
         if tk.polls_left > 0:
             return False
         self.running.remove(tk)
-        if tk.outcome == "timeout":
-            tk.n_iters, tk.passes = -1, 0
+        # the served kernel's result codes: -1 timeout, -2 non-finite result,
+        # -3 non-finite objective at the initial guess
+        codes = {"timeout": -1, "nan": -2, "bad_start": -3}
+        if tk.outcome in codes:
+            tk.n_iters, tk.passes = codes[tk.outcome], 1
         else:
-            tk.u = tk.u + 1.0 if tk.outcome == "ok" else tk.u * float("nan")
+            tk.u = tk.u + 1.0
             tk.n_iters, tk.passes = 3, 4
         return True
 
@@ -148,15 +152,17 @@ def test_warm_start_chains_frames_of_a_stream():
 
 
 def test_failed_solves_keep_the_previous_warm_start():
-    srv, sess, _ = _server()
+    srv, sess, _ = _server(max_queue=8)
     assert srv.add_stream("a", dict(n_nodes=2), REF)
     r1 = srv.submit("a", _frame())
     r2 = srv.submit("a", _frame(-1.0))  # the stand-in reports a barrier time-out
-    r3 = srv.submit("a", _frame(-2.0))  # the stand-in returns NaN displacements
+    r3 = srv.submit("a", _frame(-2.0))  # the stand-in reports a non-finite result
+    r5 = srv.submit("a", _frame(-3.0))  # ... a non-finite objective at the initial guess
     r4 = srv.submit("a", _frame())
     srv.drain()
-    assert [r.status for r in (r1, r2, r3, r4)] == [S.DONE, S.FAILED, S.FAILED, S.DONE]
-    assert "time" in r2.reason and "non-finite" in r3.reason
+    assert [r.status for r in (r1, r2, r3, r5, r4)] == [S.DONE, S.FAILED, S.FAILED, S.REJECTED_INVALID, S.DONE]
+    assert "time" in r2.reason and "non-finite" in r3.reason and "NaN or Inf" in r5.reason
+    assert all(r.u is None for r in (r2, r3, r5))
     # r2..r4 all started from r1's result: failures did not poison the stream
     for _, _, u_in in sess.submitted[1:]:
         np.testing.assert_array_equal(u_in.numpy(), np.ones((2, 2), np.float32))
