@@ -1238,6 +1238,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     unsigned long long seen_bell = ~0ull;  // served mode, master wave: last doorbell scanned
     uint64_t idle_t0 = wall_clock64();     // served mode, master wave: start of the current idle spell
 
+    // single problem: the wave's tile range is fixed while the tile count is,
+    // so the static records of its first three tiles (node ids, bounding box,
+    // geometry) are fetched once and kept in LDS across steps
+    __shared__ int s_idc[kSingle ? kWavesPerBlock : 1][3][kWave];
+    int idc_nT = -1;
     for (int step = 0; step < max_steps; ++step) {
         const int par = step & 1, buf = step % 3;
         if (!sh && !s_hdr[2]) break;
@@ -1264,8 +1269,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 // prologue: tile t0 resolved (record, geometry, box loads, first
                 // chunk); t0+1 values and t0+2 node ids in flight
                 float2 uc[4];
-                const float tv_cur =
-                    tile_vals(descs + pa, tile_ids(descs + pa, t0 - s_toff[pa], lane), lane, s_flags[pa]);
+                int id0, id1 = 0, id2 = 0;
+                if constexpr (kSingle) {
+                    int(&ic)[3][kWave] = s_idc[wave];
+                    if (nT != idc_nT) {
+                        ic[0][lane] = tile_ids(descs, t0, lane);
+                        ic[1][lane] = t0 + 1 < t1 ? tile_ids(descs, t0 + 1, lane) : 0;
+                        ic[2][lane] = t0 + 2 < t1 ? tile_ids(descs, t0 + 2, lane) : 0;
+                        idc_nT = nT;
+                    }
+                    id0 = ic[0][lane];
+                    id1 = ic[1][lane];
+                    id2 = ic[2][lane];
+                } else {
+                    id0 = tile_ids(descs + pa, t0 - s_toff[pa], lane);
+                }
+                const float tv_cur = tile_vals(descs + pa, id0, lane, s_flags[pa]);
                 tile_uc(tv_cur, s_flags[pa], s_alpha[pa], s_beta[pa], uc);
                 TileBox b_cur = tile_box(tv_cur, uc);
                 float R[kBoxRegs];
@@ -1276,11 +1295,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 int id_n2 = 0;
                 if (t0 + 1 < t1) {
                     pb = next_prob(pa, t0 + 1);
-                    tv_n1 = tile_vals(descs + pb, tile_ids(descs + pb, t0 + 1 - s_toff[pb], lane), lane, s_flags[pb]);
+                    if constexpr (!kSingle) id1 = tile_ids(descs + pb, t0 + 1 - s_toff[pb], lane);
+                    tv_n1 = tile_vals(descs + pb, id1, lane, s_flags[pb]);
                     pc = pb;
                     if (t0 + 2 < t1) {
                         pc = next_prob(pb, t0 + 2);
-                        id_n2 = tile_ids(descs + pc, t0 + 2 - s_toff[pc], lane);
+                        id_n2 = kSingle ? id2 : tile_ids(descs + pc, t0 + 2 - s_toff[pc], lane);
                     }
                 }
                 for (int it = t0; it < t1; ++it) {
