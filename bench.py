@@ -38,9 +38,13 @@ def stages_leg(mesh, assets, mask, ref, de, reps=5, win=41, search=4):
     median of ``reps`` calls after one untimed call (device-synchronised wall
     time): pixel assets from the mesh (device kernels plus the host spring
     graph), ZNCC at every element centre, IDW of the centre shifts to the
-    nodes, Green-Lagrange LSQ strain at the nodes.  ZNCC is priced as 5 FP32
-    flops per window pixel per candidate (sums of g, g^2, f g) against the
-    FP32 vector peak."""
+    nodes, Green-Lagrange LSQ strain at the nodes.  ZNCC is reported as window
+    pixels x candidates per second and as the share of the chip's VALU issue
+    rate it implies: k_zncc issues ~22 VALU instructions per window pixel per
+    candidate in its three passes (mean, variance, and the correlation with
+    the reference's per-pixel divide; FP64 accumulation), counted in its
+    gfx950 assembly; issue rate = 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per
+    wave64 instruction x 64 lanes."""
     import torch
 
     from d2ic_amd.dic_init_motion import project_centers_to_nodes_idw
@@ -65,10 +69,13 @@ def stages_leg(mesh, assets, mask, ref, de, reps=5, win=41, search=4):
     ms_idw, u_n = timed(lambda: project_centers_to_nodes_idw(mesh.nodes_xy, c, u_c))
     nodes = torch.as_tensor(np.asarray(mesh.nodes_xy), dtype=torch.float32, device="cuda")
     ms_st, _ = timed(lambda: strain_voigt(u_n, nodes, assets.node_neighbor_index, assets.node_neighbor_degree))
-    zn_flop = 5.0 * (win * win) * (2 * search + 1) ** 2 * len(c)
+    zn_pc = float(win * win) * (2 * search + 1) ** 2 * len(c)
+    valu_lane_ops_peak = 256 * 4 * 2.4e9 / 4 * 64
+    zn_rate = zn_pc / (ms_zn * 1e-3)
     return dict(pixel_assets_ms=ms_pa, n_pixels=int(pa.pixel_coords_ref.shape[0]),
                 zncc_ms=ms_zn, zncc_centres=int(len(c)), zncc_win=win, zncc_search=search,
-                zncc_tflops=zn_flop / (ms_zn * 1e-3) / 1e12, zncc_peak_tflops=157.3,
+                zncc_pixel_candidates_per_s=zn_rate, zncc_valu_per_pixel_candidate=22,
+                zncc_valu_issue_frac=zn_rate * 22 / valu_lane_ops_peak,
                 idw_ms=ms_idw, strain_ms=ms_st, n_nodes=int(nodes.shape[0]))
 
 

@@ -112,6 +112,7 @@ __global__ __launch_bounds__(kZnccBlock) void k_zncc(const float* __restrict__ c
         double sw = 0.0;
         if (act)
             for (int i = i0; i < i1; ++i)
+#pragma unroll 4
                 for (int j = 0; j < win; ++j) sw += base[i * P + j];
         red[tid] = sw;
         __syncthreads();
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(kZnccBlock) void k_zncc(const float* __restrict__ c
         double sv = 0.0;
         if (act)
             for (int i = i0; i < i1; ++i)
+#pragma unroll 4
                 for (int j = 0; j < win; ++j) {
                     float d = base[i * P + j] - m;
                     sv += (double)(d * d);
@@ -137,6 +139,7 @@ __global__ __launch_bounds__(kZnccBlock) void k_zncc(const float* __restrict__ c
         double acc = 0.0;
         if (act)
             for (int i = i0; i < i1; ++i)
+#pragma unroll 4
                 for (int j = 0; j < win; ++j) acc += (double)(((base[i * P + j] - m) / den) * tpl[i * win + j]);
         red[tid] = acc;
         __syncthreads();
