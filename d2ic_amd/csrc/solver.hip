@@ -231,6 +231,26 @@ __device__ __forceinline__ void st2_wt(float* p, int n, float2 v) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Load of data another block stored during this launch and published by a
+// flag the caller has just observed: an agent-scope load (sc1) is served at
+// the device coherence point, so it cannot hit a copy of the line that this
+// XCD's L2 cached before the data was written (no L2 invalidate needed).
+__device__ __forceinline__ float2 ld2_fresh(const float* p, int n) {
+    const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p) + n,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float2(__uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
+}
+
+// Publication without an L2 write-back.  Every store another block reads
+// during the launch is an agent-scope store (st_wt, st2_wt, device atomics),
+// which the L2 writes through to the coherence point and acknowledges from
+// there.  So once every wave of the block has waited for its memory counter
+// to drain (this call, before the block barrier that precedes the flag
+// store), those stores are complete at device scope and the flag cannot
+// overtake them.  A release fence would add a write-back of the XCD's whole
+// L2 (buffer_wbl2), which queues behind the other blocks' fences on that XCD.
+__device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
+
 // Search direction d = -g (restart) or -g + beta * d_prev, single rounding per
 // component so every place that forms it gets the same bits.
 __device__ __forceinline__ float2 direction(float2 g, float2 dp, float beta, bool restart) {
@@ -648,12 +668,15 @@ __device__ __forceinline__ void node_cache_fill(const ProbDesc* __restrict__ D, 
 }
 
 // Node pass of a finished served problem: copy the iterate from the plan's
-// buffer U[1] to the caller's buffer U[0].
+// buffer U[1] to the caller's buffer U[0] (written through: the host reads it
+// while the kernel still runs).
 __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
     const int n0 = item * kNodesPerItem;
     for (int i = threadIdx.x; i < 2 * kNodesPerItem; i += kBlock) {
         const int n = n0 + i / 2;
-        if (n < D->nnodes) gst(D->U[0])[2 * (size_t)n + (i & 1)] = gld(D->U[1])[2 * (size_t)n + (i & 1)];
+        if (n < D->nnodes)
+            __hip_atomic_store(D->U[0] + 2 * (size_t)n + (i & 1), gld(D->U[1])[2 * (size_t)n + (i & 1)],
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -663,8 +686,9 @@ __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
 // is accepted, and the sums |g|^2, g.(g - g_prev), g.d and the spring energy
 // at the trial point (both halves of every ordered pair, as the reference
 // sums them).  Loads go out in two rounds: the node's own values, its first
-// tile entry and neighbour; then the entry's tile partials.  Entries and
-// neighbours past the first 8 follow in order.  The first evaluation (F_INIT)
+// tile entry and neighbour; then the entry's tile partials (written in this
+// step by other blocks: coherent loads).  Entries and neighbours past the
+// first 8 follow in order.  The first evaluation (F_INIT)
 // has d = 0: the trial point is u.
 __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
                             double* s_red, int par, const NodeCache* nc) {
@@ -711,12 +735,12 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             if (!init) dj = direction(ld2(G, j), ld2(Dp, j), beta, restart);
         }
         if (ent >= 0) {
-            const float2 q = ld2(D->TP, ent);
+            const float2 q = ld2_fresh(D->TP, ent);
             sx += (double)q.x;
             sy += (double)q.y;
         }
         for (int e = e0 + sub + kLanesPerNode; e < e1; e += kLanesPerNode) {
-            const float2 q = ld2(D->TP, gld(D->node_ent)[e]);
+            const float2 q = ld2_fresh(D->TP, gld(D->node_ent)[e]);
             sx += (double)q.x;
             sy += (double)q.y;
         }
@@ -1097,9 +1121,10 @@ __device__ __forceinline__ int block_of_vb(int v, int nb) {
 
 // Block-wide: wait until every block holding a tile that touches the nodes of
 // node item `item` (tile range item_tr, offset toff in this step's global tile
-// order) has published this step's tile partials, then acquire them.  The
-// wave-0 poll is bounded like the grid barrier; on expiry (or another block's
-// error) it raises ws->err, clears *s_ok and returns false.
+// order) has published this step's tile partials (the node item reads them
+// with coherent loads, so no acquire fence follows).  The wave-0 poll is
+// bounded like the grid barrier; on expiry (or another block's error) it
+// raises ws->err, clears *s_ok and returns false.
 __device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, int nT, int nb, int nwaves, int step,
                            Workspace* __restrict__ ws, int* s_ok) {
     const int* tr = reinterpret_cast<const int*>(D->item_tr) + 2 * (size_t)item;
@@ -1136,10 +1161,7 @@ __device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, i
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        if (lane == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (!ok) *s_ok = 0;
-        }
+        if (lane == 0 && !ok) *s_ok = 0;
     }
     __syncthreads();
     return *s_ok != 0;
@@ -1424,9 +1446,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         //      for the blocks holding the tiles of its nodes ----
         const int ci0 = item_begin(vb, nC, nb), ci1 = item_begin(vb + 1, nC, nb);
         if (nC > 0) {
+            drain_stores();
             __syncthreads();
             if (threadIdx.x == 0) {
-                if (my_t0 < my_t1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 __hip_atomic_store(&ws->tiles_done[blockIdx.x * kArriveStride], (unsigned)(step + 1),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -1446,16 +1468,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         // ---- grid barrier: every block publishes its arrival; block 0's first
         //      wave waits for all of them and releases the step.  A release
         //      word carries the step and the abort / session-stop bits. ----
+        drain_stores();
         __syncthreads();
         if (threadIdx.x == 0) {
-            // Only a block that wrote data for other blocks in this step
-            // releases it: a release fence writes back its XCD's L2, and idle
-            // blocks fencing at once would queue those write-backs in front of
-            // the loads of the blocks still working on that XCD.  (An idle
-            // block's loads of this step have all returned: their values were
-            // used before the block arrived.)
-            const bool wrote = blockIdx.x == 0 || my_t0 < my_t1 || ci0 < ci1;
-            if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             s_stop = 0;
@@ -1500,7 +1515,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                __threadfence();
+                // acquire the step's data for this block's decision; a served
+                // session also releases the descriptors serve_admit writes
+                if constexpr (kServe)
+                    __threadfence();
+                else
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 bool stop = false;
                 if constexpr (kServe)
                     stop = serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
