@@ -189,9 +189,13 @@ struct Workspace {
     int bad[3][kMaxProblems];
     PlanEntry plan[kMaxProblems];  // first step's plan (written by k_init)
     int n_tiles_active, n_chunks_active, any_active, pad0;
-    unsigned arrive[kMaxBlocks * kArriveStride];   // last step each block finished (one line per block)
-    unsigned release[kMaxBlocks * kArriveStride];  // last step every block finished (one line per block)
-    unsigned tiles_done[kMaxBlocks * kArriveStride];  // last step whose tiles a block finished
+    // one 128-byte line per block: the last step the block finished (words of
+    // the tagged barrier in single-problem solves)
+    alignas(128) unsigned arrive[kMaxBlocks * kArriveStride];
+    // one 128-byte line per block: the last step every block finished (the
+    // tagged barrier's decision words in single-problem solves)
+    alignas(128) unsigned release[kMaxBlocks * kArriveStride];
+    alignas(128) unsigned tiles_done[kMaxBlocks * kArriveStride];  // last step whose tiles a block finished
     unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
     int stop;                                      // served mode: every block leaves after this barrier
     int err;
@@ -557,8 +561,9 @@ __device__ __forceinline__ void chunk_accum(const ProbDesc* __restrict__ D, cons
 // steps inside each 8-lane group (row_half_mirror, quad_perm [1,0,3,2],
 // quad_perm [2,3,0,1]) leave lane i with the group sum of value
 // 4*bit2(i) + 2*bit0(i) + bit1(i); row_ror:8 and two xor shuffles (16, 32)
-// add the 8 groups.  Lanes 0..7 store the 8 totals.  Fixed association order.
-__device__ __forceinline__ void reduce8_store(const float (&v)[8], int lane, D2IC_GLOBAL float* out) {
+// add the 8 groups.  Lanes 0..7 hold the 8 totals (value index returned in
+// *idx).  Fixed association order.
+__device__ __forceinline__ float reduce8(const float (&v)[8], int lane, int* idx) {
     const bool b2 = (lane & 4) != 0, b0 = (lane & 1) != 0, b1 = (lane & 2) != 0;
     float w[4];
 #pragma unroll
@@ -583,7 +588,15 @@ __device__ __forceinline__ void reduce8_store(const float (&v)[8], int lane, D2I
     y += dpp_f<0x128>(y);
     y += __shfl_xor(y, 16, kWave);
     y += __shfl_xor(y, 32, kWave);
-    if (lane < 8) st_wt(out + ((b2 ? 4 : 0) + (b0 ? 2 : 0) + (b1 ? 1 : 0)), y);
+    *idx = (b2 ? 4 : 0) + (b0 ? 2 : 0) + (b1 ? 1 : 0);
+    return y;
+}
+
+// reduce8, then lanes 0..7 store the totals
+__device__ __forceinline__ void reduce8_store(const float (&v)[8], int lane, D2IC_GLOBAL float* out) {
+    int idx;
+    const float y = reduce8(v, lane, &idx);
+    if (lane < 8) st_wt(out + idx, y);
 }
 
 // sum over the 8 lanes of an aligned lane group (DPP quad_perm and
@@ -603,7 +616,8 @@ __device__ __forceinline__ float group_sum(float v) {
 
 // Block-level sums of four per-lane values in fixed order: wave sums through
 // DPP, then the 16 wave partials summed across the first row of wave 0
-// (quad_perm, row_half_mirror, row_mirror); lane 0 stores them to out[0..3].
+// (quad_perm, row_half_mirror, row_mirror); lane 0 stores them to out[0..3],
+// or, with out == nullptr, to s_red[4 * kWavesPerBlock + 0..3] (LDS).
 static_assert(kWavesPerBlock <= 16, "block_store4 folds one 16-lane row");
 __device__ __forceinline__ void block_store4(const double (&in)[4], double* s_red, double* out) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -628,7 +642,12 @@ __device__ __forceinline__ void block_store4(const double (&in)[4], double* s_re
         }
         if (lane == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) st_wt(out + q, v[q]);
+            for (int q = 0; q < 4; ++q) {
+                if (out)
+                    st_wt(out + q, v[q]);
+                else
+                    s_red[4 * kWavesPerBlock + q] = v[q];
+            }
         }
     }
     __syncthreads();
@@ -680,18 +699,36 @@ __device__ void node_item_copy(const ProbDesc* __restrict__ D, int item) {
     }
 }
 
+// wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
+// a wait is abandoned only after both 2^20 polls and 4 s.
+constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
+constexpr unsigned kBarrierMinPolls = 1u << 20;
+
+__device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, int nT, int nb, int nwaves, int step,
+                           Workspace* __restrict__ ws, int* s_ok);
+
+// Arguments of the wait for the tiles of a node item's nodes (wait_tiles).
+struct TileWait {
+    int toff, nT, nb, nwaves, step;
+    Workspace* ws;
+    int* s_ok;
+};
+
 // Node item of a pass (8 lanes per node): direction d, trial point
 // u + alpha d, the gradient there (tile partials gathered through the node CSR
 // plus the spring term), written to the other buffers for the case the trial
 // is accepted, and the sums |g|^2, g.(g - g_prev), g.d and the spring energy
 // at the trial point (both halves of every ordered pair, as the reference
-// sums them).  Loads go out in two rounds: the node's own values, its first
-// tile entry and neighbour; then the entry's tile partials (written in this
-// step by other blocks: coherent loads).  Entries and neighbours past the
-// first 8 follow in order.  The first evaluation (F_INIT)
-// has d = 0: the trial point is u.
-__device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
-                            double* s_red, int par, const NodeCache* nc) {
+// sums them), stored to NCP or, with keep, left in the block's LDS for the
+// arrival words of the tagged barrier.  Loads go out in three rounds: the
+// node's own values of the previous step, its first tile entry and neighbour
+// (before the item waits for the tiles of its nodes, wait_tiles); then the
+// entry's tile partials (written in this step by other blocks: coherent
+// loads).  Entries and neighbours past the first 8 follow in order.  The first
+// evaluation (F_INIT) has d = 0: the trial point is u.  Returns false if the
+// wait for the tiles failed.
+__device__ bool node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
+                            double* s_red, int par, const NodeCache* nc, bool keep, const TileWait& tw) {
     const int cur = (flags & F_CUR) ? 1 : 0, dcur = (flags & F_DCUR) ? 1 : 0;
     const bool restart = (flags & F_RESTART) != 0, init = (flags & F_INIT) != 0;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -705,10 +742,13 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
     const float* U = init ? D->U0 : D->U[cur];
     const float* G = D->G[cur];
     const float* Dp = D->D[dcur];
+    const int md = D->nb_maxdeg;
+    int e0 = 0, e1 = 0, j = -1, ent = -1;
+    float w = 0.f;
+    // the node's and its first neighbour's values of the previous step are
+    // requested before the wait for this step's tiles, so they arrive during it
+    float2 u0 = make_float2(0.f, 0.f), dpn = u0, uj = u0, gj = u0, dpj = u0;
     if (valid) {
-        const int md = D->nb_maxdeg;
-        int e0, e1, j = -1, ent;
-        float w = 0.f;
         if (nc) {
             e0 = nc->e0[threadIdx.x];
             e1 = nc->e1[threadIdx.x];
@@ -724,16 +764,24 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             }
             ent = e0 + sub < e1 ? gld(D->node_ent)[e0 + sub] : -1;
         }
-        const float2 u0 = ld2(U, n);
+        u0 = ld2(U, n);
         if (!init) {
             go = ld2(G, n);
-            dn = direction(go, ld2(Dp, n), beta, restart);
+            dpn = ld2(Dp, n);
         }
-        float2 uj = make_float2(0.f, 0.f), dj = uj;
         if (j >= 0) {
             uj = ld2(U, j);
-            if (!init) dj = direction(ld2(G, j), ld2(Dp, j), beta, restart);
+            if (!init) {
+                gj = ld2(G, j);
+                dpj = ld2(Dp, j);
+            }
         }
+    }
+    if (!wait_tiles(D, item, tw.toff, tw.nT, tw.nb, tw.nwaves, tw.step, tw.ws, tw.s_ok)) return false;
+    if (valid) {
+        if (!init) dn = direction(go, dpn, beta, restart);
+        float2 dj = make_float2(0.f, 0.f);
+        if (j >= 0 && !init) dj = direction(gj, dpj, beta, restart);
         if (ent >= 0) {
             const float2 q = ld2_fresh(D->TP, ent);
             sx += (double)q.x;
@@ -779,7 +827,30 @@ __device__ void node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
         }
     }
     const double sums[4] = {gg, gnum, gdot, E};
-    block_store4(sums, s_red, D->NCP + 4 * ((size_t)par * D->nchunks + item));
+    block_store4(sums, s_red, keep ? nullptr : D->NCP + 4 * ((size_t)par * D->nchunks + item));
+    return true;
+}
+
+// Global copy of a problem state, stored through the global address space:
+// a flat store would count against the LDS counter, so the wave's next LDS
+// wait would also wait for the store to reach memory.
+__device__ __forceinline__ void store_state(ProbState* S, const ProbState& st) {
+    D2IC_GLOBAL ProbState* g = gst(S);
+    g->mode = st.mode;
+    g->k = st.k;
+    g->trial = st.trial;
+    g->restart = st.restart;
+    g->final_eval = st.final_eval;
+    g->cur = st.cur;
+    g->dcur = st.dcur;
+    g->passes = st.passes;
+    g->alpha = st.alpha;
+    g->beta = st.beta;
+    g->gnorm = st.gnorm;
+    g->Jtrial = st.Jtrial;
+    g->J = st.J;
+    g->gg = st.gg;
+    g->gd = st.gd;
 }
 
 // State transition of one problem after a pass (run identically in every
@@ -802,9 +873,9 @@ __device__ int decide(const DecideDesc& D, ProbState& st, double sr2, bool bad, 
     } else {
         if (!init) {
             if (D.hist && writer) {
-                D.hist[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
-                D.hist[3 * st.k + 1] = st.gnorm;
-                D.hist[3 * st.k + 2] = st.alpha;
+                gst(D.hist)[3 * st.k] = st.final_eval ? st.Jtrial : (float)Jc;
+                gst(D.hist)[3 * st.k + 1] = st.gnorm;
+                gst(D.hist)[3 * st.k + 2] = st.alpha;
             }
             st.k += 1;
         }
@@ -816,15 +887,15 @@ __device__ int decide(const DecideDesc& D, ProbState& st, double sr2, bool bad, 
         if (st.k >= D.max_iter) {
             // while_loop exits on k < max_iter before evaluating anything
             st.mode = MODE_DONE;
-            if (writer) *D.n_iters = st.k;
+            if (writer) *gst(D.n_iters) = st.k;
         } else if (gnorm < D.tol) {
             st.mode = MODE_DONE;
             if (D.hist && writer) {
-                D.hist[3 * st.k] = (float)st.J;
-                D.hist[3 * st.k + 1] = gnorm;
-                D.hist[3 * st.k + 2] = 0.f;
+                gst(D.hist)[3 * st.k] = (float)st.J;
+                gst(D.hist)[3 * st.k + 1] = gnorm;
+                gst(D.hist)[3 * st.k + 2] = 0.f;
             }
-            if (writer) *D.n_iters = st.k;
+            if (writer) *gst(D.n_iters) = st.k;
         } else {
             float b = 0.f;
             if (st.k > 0) {
@@ -974,7 +1045,7 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
             s_st[p] = st;
-            if (writer) S[p] = st;
+            if (writer) store_state(S + p, st);
             s_flags[p] = fl2;
             s_alpha[p] = st.alpha;
             s_beta[p] = st.beta;
@@ -1097,10 +1168,66 @@ __device__ __forceinline__ int item_begin(int vb, int nC, int nb) {
     return (int)(((long long)vb * nC + nb - 1) / nb);
 }
 
-// wall_clock64() runs at 100 MHz on MI355X (hipDeviceAttributeWallClockRate);
-// a wait is abandoned only after both 2^20 polls and 4 s.
-constexpr uint64_t kBarrierTimeoutTicks = 400000000ull;
-constexpr unsigned kBarrierMinPolls = 1u << 20;
+
+// Virtual block of hardware block b: consecutive hardware blocks sit on
+// different XCDs, so each XCD gets one contiguous range of virtual blocks.
+__device__ __forceinline__ int vb_of(int b, int nb) {
+    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
+// ---------------------------------------------------------------------------
+// Tagged barrier of single-problem solves.  A block's arrival line carries,
+// besides the arrival itself, what the step's decision needs from the block:
+// its fixed-point residual sums (hi, lo), its non-finite flag and the four
+// sums of its node item; the release line carries the decision (next flags,
+// alpha, beta).  Each 64-bit word holds (step + 1) mod 4 in its top two bits
+// and 62 payload bits, and a reader takes a line once every word carries the
+// current tag.  Only the previous step's words can still be there (a block
+// arrives at step s + 1 only after the release of step s, and the master
+// releases step s + 1 only after every arrival of s + 1), so two tag bits
+// tell them apart; every word is one single-copy-atomic 64-bit store, so a
+// word with the current tag holds the current payload.  The master (block 0,
+// wave 0) thus has every input of the decision once it has seen the last
+// arrival, runs the state machine once, and the blocks read the outcome in
+// their release poll: no load round trip after the barrier, no global
+// atomics for the residual sums and no NCP stores.
+// ---------------------------------------------------------------------------
+constexpr int kArrWords = 7;  // hi, lo, 4 doubles, the non-finite flag: 385 bits in 7 x 62
+// Release word of the tagged barrier: beta (bits 0-31), the next flags
+// (32-39), the biased exponent of alpha (40-47; alpha is 0 or 2^-k, k <= 10,
+// in a single-problem solve: 1 halved per rejected trial), abort (48), tag
+// (62-63).
+constexpr int kRelFlagsShift = 32, kRelAlphaShift = 40;
+constexpr unsigned long long kRelAbort = 1ull << 48;
+constexpr unsigned long long kPayMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ unsigned long long step_tag(int step) {
+    return (unsigned long long)((step + 1) & 3) << 62;
+}
+
+// x[0..6] as one little-endian bit stream; word i carries bits [62 i, 62 i + 62)
+__device__ __forceinline__ void pack62(const unsigned long long (&x)[kArrWords], unsigned long long tag,
+                                       unsigned long long (&w)[kArrWords]) {
+#pragma unroll
+    for (int i = 0; i < kArrWords; ++i) {
+        const int off = 62 * i, j = off / 64, sh = off % 64;
+        unsigned long long v = x[j] >> sh;
+        if (sh > 2 && j + 1 < kArrWords) v |= x[j + 1] << (64 - sh);
+        w[i] = tag | (v & kPayMask);
+    }
+}
+
+// inverse of pack62 (the last value keeps only the 50 bits the words hold)
+__device__ __forceinline__ void unpack62(const unsigned long long (&w)[kArrWords], unsigned long long (&x)[kArrWords]) {
+#pragma unroll
+    for (int j = 0; j < kArrWords; ++j) {
+        const int off = 64 * j, i = off / 62, sh = off % 62;
+        unsigned long long v = (w[i] & kPayMask) >> sh;
+        if (i + 1 < kArrWords) v |= (w[i + 1] & kPayMask) << (62 - sh);
+        x[j] = v;
+    }
+}
+
 
 // Virtual block holding global tile gt: wave w takes tiles
 // [nT w / nwaves, nT (w + 1) / nwaves), so gt belongs to
@@ -1167,6 +1294,214 @@ __device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, i
     return *s_ok != 0;
 }
 
+
+// Next-step plan of a single problem in the block's LDS (what block_decide
+// derives for P = 1).
+#define D2IC_LDS __attribute__((address_space(3)))
+__device__ __forceinline__ void set_plan1(const D2IC_LDS DecideDesc* s_dd, int fl, float alpha, float beta,
+                                          D2IC_LDS int* s_flags, D2IC_LDS float* s_alpha, D2IC_LDS float* s_beta,
+                                          D2IC_LDS int* s_hdr, D2IC_LDS unsigned long long* s_hi,
+                                          D2IC_LDS unsigned long long* s_lo, D2IC_LDS int* s_bad) {
+    s_flags[0] = fl;
+    s_alpha[0] = alpha;
+    s_beta[0] = beta;
+    s_hdr[0] = (fl & F_DOA) ? s_dd[0].ntiles : 0;
+    s_hdr[1] = (fl & (F_DOB | F_COPY)) ? s_dd[0].nchunks : 0;
+    s_hdr[2] = fl != 0;  // decide() returns no flags exactly when the problem is done
+    s_hi[0] = 0ull;
+    s_lo[0] = 0ull;
+    s_bad[0] = 0;
+}
+
+// Tagged barrier, block 0 (every thread calls it): waves 0..3 wait for the
+// arrival words of blocks 64 w + lane; their residual sums (integers: any
+// order) and non-finite flags are combined through LDS, the node items' sums
+// are placed in LDS by item and added in block_decide's order (item
+// c = lane + 64 j added in j order, then the wave sum); wave 0 runs the state
+// machine and releases the next step's plan to every block.  s_items: LDS
+// scratch of 4 doubles per node item; s_part: LDS scratch of 16 words.  Kept
+// out of line (inlined, it changes the register allocation of the tile
+// pipeline); its LDS and global arguments are typed so that its accesses are
+// LDS and global instructions rather than flat ones.
+__device__ __noinline__ void tagged_master(ProbState* __restrict__ S, D2IC_GLOBAL Workspace* ws, int step, int nb,
+                                           int nC, const D2IC_LDS DecideDesc* s_dd, D2IC_LDS ProbState* s_st,
+                                           D2IC_LDS int* s_flags, D2IC_LDS float* s_alpha, D2IC_LDS float* s_beta,
+                                           D2IC_LDS int* s_hdr, D2IC_LDS unsigned long long* s_hi,
+                                           D2IC_LDS unsigned long long* s_lo, D2IC_LDS int* s_bad,
+                                           D2IC_LDS double* s_items, D2IC_LDS unsigned long long* s_part,
+                                           D2IC_LDS int* s_ok) {
+    static_assert(4 * kWave >= 256 && kWavesPerBlock >= 4, "four polling waves cover 256 blocks");
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const unsigned long long tag = step_tag(step);
+    if (wave < 4) {
+        const int b = wave * kWave + lane;
+        const D2IC_GLOBAL unsigned long long* aw =
+            reinterpret_cast<const D2IC_GLOBAL unsigned long long*>(&ws->arrive[(b < nb ? b : 0) * kArriveStride]);
+        const uint64_t tstart = wall_clock64();
+        unsigned polls = 0;
+        int ok = 1;
+        unsigned long long w[kArrWords];
+        while (true) {
+            // every load of one poll is in flight at once
+            const int err = __hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int i = 0; i < kArrWords; ++i)
+                w[i] = b < nb ? __hip_atomic_load(aw + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tag;
+            bool mine = true;
+#pragma unroll
+            for (int i = 0; i < kArrWords; ++i) mine = mine & ((w[i] & ~kPayMask) == tag);
+            if (__all(mine)) break;
+            if (err) {
+                ok = 0;
+                break;
+            }
+            ++polls;
+            if (polls > kBarrierMinPolls && wall_clock64() - tstart > kBarrierTimeoutTicks) {
+                if (lane == 0) {
+                    __hip_atomic_exchange(&ws->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    printf("d2ic: grid barrier timeout (arrivals, step %d)\n", step);
+                }
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        unsigned long long hs = 0ull, ls = 0ull;
+        bool bad = false;
+        if (ok && b < nb) {
+            unsigned long long x[kArrWords];
+            unpack62(w, x);
+            hs = x[0];
+            ls = x[1];
+            bad = (x[6] & 1ull) != 0;
+            const int v = vb_of(b, nb), c0 = item_begin(v, nC, nb);
+            if (item_begin(v + 1, nC, nb) > c0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s_items[4 * c0 + q] = __longlong_as_double((long long)x[2 + q]);
+            }
+        }
+        hs = wave_sum63(hs);
+        ls = wave_sum63(ls);
+        bad = __any(bad);
+        if (lane == kWave - 1) {
+            s_part[3 * wave] = hs;
+            s_part[3 * wave + 1] = ls;
+            s_part[3 * wave + 2] = (bad ? 1ull : 0ull) | (ok ? 0ull : 2ull);
+        }
+    }
+    // acquire the step's node arrays for this block's next loads
+    if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    if (wave != 0) return;
+    unsigned long long hs = 0ull, ls = 0ull, fl_bits = 0ull;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        hs += s_part[3 * k];
+        ls += s_part[3 * k + 1];
+        fl_bits |= s_part[3 * k + 2];
+    }
+    int ok = (fl_bits & 2ull) ? 0 : 1;
+    int fl = 0;
+    float alpha = 0.f, beta = 0.f;
+    if (ok) {
+        // nC <= 4 * kWave: one round of block_decide's loop; every load is
+        // issued (inside the scratch), entries past nC are replaced by zeros
+        double sg = 0.0, sn = 0.0, sd = 0.0, sE = 0.0;
+        double v[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[u][q] = s_items[4 * (u * kWave + lane) + q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool in = u * kWave + lane < nC;
+            sg += in ? v[u][0] : 0.0;
+            sn += in ? v[u][1] : 0.0;
+            sd += in ? v[u][2] : 0.0;
+            sE += in ? v[u][3] : 0.0;
+        }
+        sg = wave_sum63(sg);
+        sn = wave_sum63(sn);
+        sd = wave_sum63(sd);
+        sE = wave_sum63(sE);
+        if (lane == kWave - 1) {
+            const double sr2 = (double)hs + (double)ls / kFixScale;
+            // (generic views of the LDS copies: the compiler infers them back to LDS)
+            ProbState st = *(const ProbState*)s_st;
+            const DecideDesc dd = *(const DecideDesc*)s_dd;
+            fl = decide(dd, st, sr2, (fl_bits & 1ull) != 0, sg, sn, sd, sE, true);
+            *(ProbState*)s_st = st;
+            store_state(S, st);
+            alpha = st.alpha;
+            beta = st.beta;
+        }
+        fl = __builtin_amdgcn_readlane(fl, kWave - 1);
+        alpha = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(alpha), kWave - 1));
+        beta = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(beta), kWave - 1));
+        if (__float_as_uint(alpha) & 0x807fffffu) {  // not 0 or a power of two: cannot travel in the word
+            if (lane == 0) {
+                __hip_atomic_exchange(&ws->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                printf("d2ic: step length %g is not a power of two (step %d)\n", (double)alpha, step);
+            }
+            ok = 0;
+        }
+    }
+    const unsigned long long r0 = tag | (ok ? 0ull : kRelAbort) |
+                                  ((unsigned long long)((__float_as_uint(alpha) >> 23) & 0xffu) << kRelAlphaShift) |
+                                  ((unsigned long long)(unsigned)(fl & 0xff) << kRelFlagsShift) | __float_as_uint(beta);
+    for (int b = lane; b < nb; b += kWave) {
+        D2IC_GLOBAL unsigned long long* rw =
+            reinterpret_cast<D2IC_GLOBAL unsigned long long*>(&ws->release[b * kArriveStride]);
+        __hip_atomic_store(rw, r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) {
+        if (ok)
+            set_plan1(s_dd, fl, alpha, beta, s_flags, s_alpha, s_beta, s_hdr, s_hi, s_lo, s_bad);
+        else
+            *s_ok = 0;
+    }
+}
+
+// Tagged barrier, first thread of every other block: wait for the release
+// words of this step and take the plan they carry (out of line, typed
+// arguments: see tagged_master).
+__device__ __noinline__ void tagged_wait(D2IC_GLOBAL Workspace* ws, int step, const D2IC_LDS DecideDesc* s_dd,
+                                         D2IC_LDS int* s_flags, D2IC_LDS float* s_alpha, D2IC_LDS float* s_beta,
+                                         D2IC_LDS int* s_hdr, D2IC_LDS unsigned long long* s_hi,
+                                         D2IC_LDS unsigned long long* s_lo, D2IC_LDS int* s_bad, D2IC_LDS int* s_ok) {
+    const unsigned long long tag = step_tag(step);
+    const D2IC_GLOBAL unsigned long long* rw =
+        reinterpret_cast<const D2IC_GLOBAL unsigned long long*>(&ws->release[blockIdx.x * kArriveStride]);
+    const uint64_t tstart = wall_clock64();
+    unsigned polls = 0;
+    unsigned long long r = 0ull;
+    bool abort = false;
+    while (true) {
+        r = __hip_atomic_load(rw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int err = __hip_atomic_load(&ws->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((r & ~kPayMask) == tag) break;
+        if (err) {
+            abort = true;
+            break;
+        }
+        ++polls;
+        if (polls > kBarrierMinPolls && wall_clock64() - tstart > kBarrierTimeoutTicks) {
+            __hip_atomic_exchange(&ws->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            printf("d2ic: grid barrier timeout (block %u, step %d)\n", blockIdx.x, step);
+            abort = true;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (abort || (r & kRelAbort)) {
+        *s_ok = 0;
+        return;
+    }
+    set_plan1(s_dd, (int)((r >> kRelFlagsShift) & 0xffull), __uint_as_float((unsigned)((r >> kRelAlphaShift) & 0xffull) << 23),
+              __uint_as_float((unsigned)r), s_flags, s_alpha, s_beta, s_hdr, s_hi, s_lo, s_bad);
+}
+
 // sh == nullptr: a fixed batch of P problems (d2ic_cg_solve_batch).  Otherwise a
 // served session over P <= kMaxSlots slots whose requests arrive through the
 // mailbox; the kernel runs until it has been idle for sh->idle_ticks or the
@@ -1201,7 +1536,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ int s_hdr[3];
     __shared__ int s_ok;
     __shared__ int s_stop;
-    __shared__ double s_red[4 * kWavesPerBlock];
+    __shared__ double s_red[4 * kWavesPerBlock + 4];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
     __shared__ typename std::conditional<kSingle, NodeCache, int>::type s_nc_store;
     NodeCache* s_nc = kSingle ? reinterpret_cast<NodeCache*>(&s_nc_store) : nullptr;
@@ -1212,11 +1547,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     const int nb = gridDim.x;
     // XCD-aware numbering: consecutive blocks sit on different XCDs, so give each
     // XCD one contiguous share of the tiles (and of the image).
-    const int vb = (nb % 8 == 0) ? (int)((blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8) : (int)blockIdx.x;
+    const int vb = vb_of((int)blockIdx.x, nb);
     const int gwave = __builtin_amdgcn_readfirstlane(vb * kWavesPerBlock + wave);
     const int nwaves = nb * kWavesPerBlock;
 
     const ProbDesc* descs = gdesc;
+    // tagged barrier: single problem, at most one node item per block and at
+    // most four blocks per lane of the master wave
+    const bool tagged = kSingle && nb <= 4 * kWave && gdesc[0].nchunks <= nb;
     if constexpr (kServe) {
         constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
         for (int i = threadIdx.x; i < P * kWords; i += kBlock)
@@ -1431,7 +1769,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             }
         }
         __syncthreads();
-        {
+        if (!tagged) {
             const int slot = blockIdx.x % kSlots;
             for (int p = threadIdx.x; p < P; p += kBlock) {
                 if (s_hi[p]) atomicAdd(&ws->hi[buf][p][slot], s_hi[p]);
@@ -1461,8 +1799,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 node_item_copy(D, c - s_coff[p]);
                 continue;
             }
-            if (!wait_tiles(D, c - s_coff[p], s_toff[p], nT, nb, nwaves, step, ws, &s_ok)) break;
-            node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par, c == nc_item ? s_nc : nullptr);
+            const TileWait tw{s_toff[p], nT, nb, nwaves, step, ws, &s_ok};
+            if (!node_item_b(D, c - s_coff[p], fl, s_alpha[p], s_beta[p], s_red, par, c == nc_item ? s_nc : nullptr,
+                             tagged, tw))
+                break;
         }
 
         // ---- grid barrier: every block publishes its arrival; block 0's first
@@ -1471,11 +1811,39 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         drain_stores();
         __syncthreads();
         if (threadIdx.x == 0) {
-            __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (tagged) {
+                unsigned long long x[kArrWords], w[kArrWords];
+                x[0] = s_hi[0];
+                x[1] = s_lo[0];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    x[2 + q] = ci1 > ci0 ? (unsigned long long)__double_as_longlong(s_red[4 * kWavesPerBlock + q]) : 0ull;
+                x[6] = s_bad[0] ? 1ull : 0ull;
+                pack62(x, step_tag(step), w);
+                unsigned long long* aw = reinterpret_cast<unsigned long long*>(&ws->arrive[blockIdx.x * kArriveStride]);
+#pragma unroll
+                for (int i = 0; i < kArrWords; ++i)
+                    __hip_atomic_store(aw + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&ws->arrive[blockIdx.x * kArriveStride], (unsigned)(step + 1), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
             s_stop = 0;
         }
-        if (blockIdx.x == 0) {
+        if (tagged) {
+            if (blockIdx.x == 0) {
+                tagged_master(S, gst(ws), step, nb, nC, (const D2IC_LDS DecideDesc*)s_dd, (D2IC_LDS ProbState*)s_st,
+                              (D2IC_LDS int*)s_flags, (D2IC_LDS float*)s_alpha, (D2IC_LDS float*)s_beta,
+                              (D2IC_LDS int*)s_hdr, (D2IC_LDS unsigned long long*)s_hi,
+                              (D2IC_LDS unsigned long long*)s_lo, (D2IC_LDS int*)s_bad,
+                              (D2IC_LDS double*)&s_box[0][0], (D2IC_LDS unsigned long long*)s_red, (D2IC_LDS int*)&s_ok);
+            } else if (threadIdx.x == 0) {
+                tagged_wait(gst(ws), step, (const D2IC_LDS DecideDesc*)s_dd, (D2IC_LDS int*)s_flags,
+                            (D2IC_LDS float*)s_alpha, (D2IC_LDS float*)s_beta, (D2IC_LDS int*)s_hdr,
+                            (D2IC_LDS unsigned long long*)s_hi, (D2IC_LDS unsigned long long*)s_lo,
+                            (D2IC_LDS int*)s_bad, (D2IC_LDS int*)&s_ok);
+            }
+        } else if (blockIdx.x == 0) {
             if (wave == 0) {
                 const uint64_t tstart = wall_clock64();
                 unsigned polls = 0;
@@ -1565,8 +1933,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         if (!s_ok) break;
         if (s_stop) break;
         if constexpr (kServe) serve_refresh(gdesc, s_desc, s_dd, ws);
-        block_decide(s_dd, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_hi, s_lo,
-                     s_bad);
+        if (!tagged)
+            block_decide(s_dd, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_hi, s_lo,
+                         s_bad);
     }
     if (sh && blockIdx.x == 0 && threadIdx.x == 0) {
         __threadfence_system();
