@@ -192,8 +192,9 @@ struct Workspace {
     // one 128-byte line per block: the last step the block finished (words of
     // the tagged barrier in single-problem solves)
     alignas(128) unsigned arrive[kMaxBlocks * kArriveStride];
-    // one 128-byte line per block: the last step every block finished (the
-    // tagged barrier's decision words in single-problem solves)
+    // one 128-byte line per block: the last step every block finished (in
+    // single-problem solves, lines 0..7 hold the tagged barrier's release
+    // word, one line per XCD)
     alignas(128) unsigned release[kMaxBlocks * kArriveStride];
     alignas(128) unsigned tiles_done[kMaxBlocks * kArriveStride];  // last step whose tiles a block finished
     unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
@@ -347,7 +348,8 @@ __device__ __forceinline__ float tile_vals(const ProbDesc* __restrict__ D, int i
         const int which = lane >> 3;
         const float* base = which == 0 ? ((flags & F_INIT) ? D->U0 : D->U[(flags & F_CUR) ? 1 : 0])
                                        : (which == 1 ? D->G[(flags & F_CUR) ? 1 : 0] : D->D[(flags & F_DCUR) ? 1 : 0]);
-        return gld(base)[2 * (size_t)id + (lane & 1)];
+        // written by other blocks' node items during the launch: coherent load
+        return __hip_atomic_load(base + 2 * (size_t)id + (lane & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return __int_as_float(id);
 }
@@ -723,8 +725,8 @@ struct TileWait {
 // arrival words of the tagged barrier.  Loads go out in three rounds: the
 // node's own values of the previous step, its first tile entry and neighbour
 // (before the item waits for the tiles of its nodes, wait_tiles); then the
-// entry's tile partials (written in this step by other blocks: coherent
-// loads).  Entries and neighbours past the first 8 follow in order.  The first
+// entry's tile partials.  Node values and tile partials are written by other
+// blocks during the launch and read with coherent loads.  Entries and neighbours past the first 8 follow in order.  The first
 // evaluation (F_INIT) has d = 0: the trial point is u.  Returns false if the
 // wait for the tiles failed.
 __device__ bool node_item_b(const ProbDesc* __restrict__ D, int item, int flags, float alpha, float beta,
@@ -764,16 +766,16 @@ __device__ bool node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             }
             ent = e0 + sub < e1 ? gld(D->node_ent)[e0 + sub] : -1;
         }
-        u0 = ld2(U, n);
+        u0 = ld2_fresh(U, n);
         if (!init) {
-            go = ld2(G, n);
-            dpn = ld2(Dp, n);
+            go = ld2_fresh(G, n);
+            dpn = ld2_fresh(Dp, n);
         }
         if (j >= 0) {
-            uj = ld2(U, j);
+            uj = ld2_fresh(U, j);
             if (!init) {
-                gj = ld2(G, j);
-                dpj = ld2(Dp, j);
+                gj = ld2_fresh(G, j);
+                dpj = ld2_fresh(Dp, j);
             }
         }
     }
@@ -798,8 +800,8 @@ __device__ bool node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
                 j = gld(D->nb_idx)[(size_t)n * md + k];
                 if (j < 0) continue;
                 w = gld(D->nb_wp)[(size_t)n * md + k];
-                uj = ld2(U, j);
-                if (!init) dj = direction(ld2(G, j), ld2(Dp, j), beta, restart);
+                uj = ld2_fresh(U, j);
+                if (!init) dj = direction(ld2_fresh(G, j), ld2_fresh(Dp, j), beta, restart);
             } else if (j < 0) {
                 continue;
             }
@@ -1190,7 +1192,12 @@ __device__ __forceinline__ int vb_of(int b, int nb) {
 // wave 0) thus has every input of the decision once it has seen the last
 // arrival, runs the state machine once, and the blocks read the outcome in
 // their release poll: no load round trip after the barrier, no global
-// atomics for the residual sums and no NCP stores.
+// atomics for the residual sums and no NCP stores.  The release goes to one
+// line per XCD (block b polls line b mod 8).  No acquire fence follows
+// either poll: every load of data that other blocks write during the launch
+// (node values, tile partials, barrier words) is a coherent agent-scope load,
+// so no L2 invalidate is needed and the XCD's L2 keeps the image and the f
+// samples from one step to the next.
 // ---------------------------------------------------------------------------
 constexpr int kArrWords = 7;  // hi, lo, 4 doubles, the non-finite flag: 385 bits in 7 x 62
 // Release word of the tagged barrier: beta (bits 0-31), the next flags
@@ -1389,8 +1396,6 @@ __device__ __noinline__ void tagged_master(ProbState* __restrict__ S, D2IC_GLOBA
             s_part[3 * wave + 2] = (bad ? 1ull : 0ull) | (ok ? 0ull : 2ull);
         }
     }
-    // acquire the step's node arrays for this block's next loads
-    if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __syncthreads();
     if (wave != 0) return;
     unsigned long long hs = 0ull, ls = 0ull, fl_bits = 0ull;
@@ -1449,9 +1454,9 @@ __device__ __noinline__ void tagged_master(ProbState* __restrict__ S, D2IC_GLOBA
     const unsigned long long r0 = tag | (ok ? 0ull : kRelAbort) |
                                   ((unsigned long long)((__float_as_uint(alpha) >> 23) & 0xffu) << kRelAlphaShift) |
                                   ((unsigned long long)(unsigned)(fl & 0xff) << kRelFlagsShift) | __float_as_uint(beta);
-    for (int b = lane; b < nb; b += kWave) {
+    if (lane < 8) {  // one release line per XCD
         D2IC_GLOBAL unsigned long long* rw =
-            reinterpret_cast<D2IC_GLOBAL unsigned long long*>(&ws->release[b * kArriveStride]);
+            reinterpret_cast<D2IC_GLOBAL unsigned long long*>(&ws->release[lane * kArriveStride]);
         __hip_atomic_store(rw, r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (lane == 0) {
@@ -1471,7 +1476,7 @@ __device__ __noinline__ void tagged_wait(D2IC_GLOBAL Workspace* ws, int step, co
                                          D2IC_LDS unsigned long long* s_lo, D2IC_LDS int* s_bad, D2IC_LDS int* s_ok) {
     const unsigned long long tag = step_tag(step);
     const D2IC_GLOBAL unsigned long long* rw =
-        reinterpret_cast<const D2IC_GLOBAL unsigned long long*>(&ws->release[blockIdx.x * kArriveStride]);
+        reinterpret_cast<const D2IC_GLOBAL unsigned long long*>(&ws->release[(blockIdx.x % 8) * kArriveStride]);
     const uint64_t tstart = wall_clock64();
     unsigned polls = 0;
     unsigned long long r = 0ull;
@@ -1493,7 +1498,6 @@ __device__ __noinline__ void tagged_wait(D2IC_GLOBAL Workspace* ws, int step, co
         }
         __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (abort || (r & kRelAbort)) {
         *s_ok = 0;
         return;
