@@ -37,6 +37,7 @@ class DICMeshBased(DICBase):
         self.solver = solver
         self.config = config
         self._state: MeshDICState | None = None
+        self._nbr_dev = None  # device copies of the strain neighbour tables (made in prepare)
 
     @property
     def state(self) -> MeshDICState | None:
@@ -57,6 +58,11 @@ class DICMeshBased(DICBase):
         nodes_dev = _lib.to_dev(assets.mesh.nodes_xy, t.float32)
         self._state = MeshDICState(ref_image=ref_dev, nodes_xy_device=nodes_dev, assets=assets,
                                    config=self.config, u0_nodal=None)
+        self._nbr_dev = None
+        if assets.node_neighbor_index is not None and assets.node_neighbor_degree is not None:
+            # one host->device copy per prepare instead of one per frame
+            self._nbr_dev = (_lib.to_dev(assets.node_neighbor_index, t.int32),
+                             _lib.to_dev(assets.node_neighbor_degree, t.int32))
         self.solver.compile(assets)
         if hasattr(self.solver, "warmup"):
             self.solver.warmup(self._state)
@@ -70,25 +76,38 @@ class DICMeshBased(DICBase):
         if self._state is None:
             raise RuntimeError("DICMeshBased.prepare() must be called before run().")
         t = _lib.torch()
-        sol = self.solver.solve(self._state, def_image)
+        # the HIP solver's launches are queued without waiting for the solve,
+        # so the strain kernel is queued behind them before the host reads
+        # the iteration count
+        launch = getattr(self.solver, "launch", None)
+        pending = launch(self._state, def_image) if launch is not None else None
+        sol = pending.result if pending is not None else self.solver.solve(self._state, def_image)
         u_nodal = sol.u_nodal
         assets = self._state.assets
         info = {"stage": "mesh_based"}
         pixel_maps = {}
+        strain = strain_info = strain_error = None
+        if assets.node_neighbor_index is not None and assets.node_neighbor_degree is not None:
+            idx, deg = self._nbr_dev if self._nbr_dev is not None else (assets.node_neighbor_index,
+                                                                       assets.node_neighbor_degree)
+            try:
+                strain = strain_voigt(u_nodal, self._state.nodes_xy_device, idx, deg,
+                                      gauge_length=self.config.strain_gauge_length, eps=self.config.strain_eps)
+                strain_info = "green_lagrange"
+            except Exception as exc:  # pragma: no cover - defensive, as in the reference
+                strain = t.zeros((u_nodal.shape[0], 3), dtype=u_nodal.dtype, device=u_nodal.device)
+                strain_info = "failed_green_lagrange"
+                strain_error = str(exc)
+        if pending is not None:
+            sol = pending.finish()
         if getattr(sol, "n_iters", None) is not None:
             info["n_iters"] = int(sol.n_iters)
         if getattr(sol, "history", None) is not None:
             info["history"] = "attached"
-        if assets.node_neighbor_index is not None and assets.node_neighbor_degree is not None:
-            try:
-                strain = strain_voigt(u_nodal, self._state.nodes_xy_device, assets.node_neighbor_index,
-                                      assets.node_neighbor_degree, gauge_length=self.config.strain_gauge_length,
-                                      eps=self.config.strain_eps)
-                info["strain"] = "green_lagrange"
-            except Exception as exc:  # pragma: no cover - defensive, as in the reference
-                strain = t.zeros((u_nodal.shape[0], 3), dtype=u_nodal.dtype, device=u_nodal.device)
-                info["strain"] = "failed_green_lagrange"
-                info["strain_error"] = str(exc)
+        if strain_info is not None:
+            info["strain"] = strain_info
+            if strain_error is not None:
+                info["strain_error"] = strain_error
         else:
             strain = t.zeros((u_nodal.shape[0], 3), dtype=u_nodal.dtype, device=u_nodal.device)
             info["strain"] = "skipped_no_neighbors"

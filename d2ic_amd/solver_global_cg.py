@@ -14,6 +14,8 @@ available) or "linear"/"bilinear" (the reference's in-repo bilinear sampler,
 """
 from __future__ import annotations
 
+import dataclasses
+
 import ctypes
 import sys
 import warnings
@@ -36,6 +38,28 @@ class GlobalCGResult:
     strain: Array
     history: Array | None = None
     n_iters: int | None = None
+
+
+class PendingSolve:
+    """A queued solve (GlobalCGSolver.launch).  ``result`` holds u (filled in
+    place by the queued kernels) and the history buffer; ``finish()`` waits
+    for the iteration count and returns the completed GlobalCGResult.  Keeps
+    the deformed image alive until then."""
+
+    def __init__(self, solver, result, hist, counters, image):
+        self._solver = solver
+        self.result = result
+        self.hist = hist
+        self.counters = counters
+        self._image = image
+        self._done = False
+
+    def finish(self) -> GlobalCGResult:
+        if not self._done:
+            self._solver._finish(self)
+            self._done = True
+            self._image = None
+        return self.result
 
 
 def _normalize_interpolation(interpolation: str) -> str:
@@ -303,6 +327,14 @@ class GlobalCGSolver(SolverBase):
             self._compiled = False
 
     def solve(self, state: Any, def_image: Array) -> GlobalCGResult:
+        res = self.launch(state, def_image).finish()
+        return dataclasses.replace(res, strain=_lib.torch().zeros_like(res.u_nodal))  # the reference's placeholder
+
+    def launch(self, state: Any, def_image: Array) -> "PendingSolve":
+        """Queue the solve on the current stream without waiting for it: the
+        returned PendingSolve holds the result arrays (u is filled in place by
+        the queued kernels; no placeholder strain) and finish() reads the
+        iteration count (the only host wait of a solve)."""
         if not self._compiled or self._plan is None:
             raise RuntimeError("GlobalCGSolver.compile() must be called before solve().")
         t = _lib.torch()
@@ -323,15 +355,19 @@ class GlobalCGSolver(SolverBase):
         _lib.call("d2ic_cg_solve", plan.handle, _lib.ptr(g), int(g.shape[0]), int(g.shape[1]), _lib.ptr(u),
                   float(cfg.reg_strength), max_iter, float(cfg.tol), _lib.ptr(hist), _lib.ptr(counters),
                   _lib.ptr(counters) + 4, _lib.stream())
-        n_iters, passes = (int(v) for v in counters.cpu().tolist())
+        save_history = bool(getattr(cfg, "save_history", False))
+        result = GlobalCGResult(u_nodal=u, strain=None, history=hist if save_history else None, n_iters=None)
+        return PendingSolve(self, result, hist, counters, g)
+
+    def _finish(self, pending: "PendingSolve") -> GlobalCGResult:
+        n_iters, passes = (int(v) for v in pending.counters.cpu().tolist())
         if n_iters < 0:
             raise RuntimeError("d2ic_cg_solve did not finish (a grid barrier timed out)")
         self.last_pass_count = passes
         if self._verbose and self._log_cg:
-            h = hist[:max(n_iters + 1, 1)].cpu().numpy()
+            h = pending.hist[:max(n_iters + 1, 1)].cpu().numpy()
             for k, (J, gn, a) in enumerate(h):
                 if np.isfinite(J):
                     print(f"CG iter {k}: J={J:.3e}, |grad|={gn:.3e}, alpha={a:.3e}")
-        save_history = bool(getattr(cfg, "save_history", False))
-        return GlobalCGResult(u_nodal=u, strain=t.zeros_like(u), history=hist if save_history else None,
-                              n_iters=n_iters)
+        pending.result = dataclasses.replace(pending.result, n_iters=n_iters)
+        return pending.result
