@@ -616,39 +616,37 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-// Block-level sums of four per-lane values in fixed order: wave sums through
-// DPP, then the 16 wave partials summed across the first row of wave 0
-// (quad_perm, row_half_mirror, row_mirror); lane 0 stores them to out[0..3],
-// or, with out == nullptr, to s_red[4 * kWavesPerBlock + 0..3] (LDS).
-static_assert(kWavesPerBlock <= 16, "block_store4 folds one 16-lane row");
-__device__ __forceinline__ void block_store4(const double (&in)[4], double* s_red, double* out) {
+// Sums over a node item of the four per-node quantities |g|^2, g.(g - g_prev),
+// g.d (held by the node's first lane) and the spring energy (one term per
+// lane), in a fixed order: the energy is first added over the node's 8 lanes,
+// the node's four values go to its LDS slot, and wave 0 adds slots lane and
+// lane + 64 and folds the wave (DPP); lane 63 stores the totals to out[0..3],
+// or, with out == nullptr, to s_red[4 * kNodesPerItem + 0..3] (LDS).
+// s_red: 4 * kNodesPerItem + 4 doubles.
+static_assert(kNodesPerItem == 2 * kWave, "wave 0 adds two node slots per lane");
+__device__ __forceinline__ void item_store4(double gg, double gnum, double gdot, double E, double* s_red,
+                                            double* out) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    double w[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = wave_sum63(in[q]);
-    if (lane == kWave - 1) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s_red[4 * wave + q] = w[q];
+    E = group_sum(E);
+    if (lane % kLanesPerNode == 0) {
+        double* slot = s_red + 4 * (wave * kNodesPerWave + lane / kLanesPerNode);
+        slot[0] = gg;
+        slot[1] = gnum;
+        slot[2] = gdot;
+        slot[3] = E;
     }
     __syncthreads();
     if (wave == 0) {
         double v[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double x = lane < kWavesPerBlock ? s_red[4 * lane + q] : 0.0;
-            x += dpp_d<0xB1>(x);
-            x += dpp_d<0x4E>(x);
-            x += dpp_d<0x141>(x);
-            x += dpp_d<0x140>(x);
-            v[q] = x;
-        }
-        if (lane == 0) {
+        for (int q = 0; q < 4; ++q) v[q] = wave_sum63(s_red[4 * lane + q] + s_red[4 * (lane + kWave) + q]);
+        if (lane == kWave - 1) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (out)
                     st_wt(out + q, v[q]);
                 else
-                    s_red[4 * kWavesPerBlock + q] = v[q];
+                    s_red[4 * kNodesPerItem + q] = v[q];
             }
         }
     }
@@ -828,8 +826,7 @@ __device__ bool node_item_b(const ProbDesc* __restrict__ D, int item, int flags,
             gdot = (double)gx * (double)dn.x + (double)gy * (double)dn.y;
         }
     }
-    const double sums[4] = {gg, gnum, gdot, E};
-    block_store4(sums, s_red, keep ? nullptr : D->NCP + 4 * ((size_t)par * D->nchunks + item));
+    item_store4(gg, gnum, gdot, E, s_red, keep ? nullptr : D->NCP + 4 * ((size_t)par * D->nchunks + item));
     return true;
 }
 
@@ -1540,7 +1537,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ int s_hdr[3];
     __shared__ int s_ok;
     __shared__ int s_stop;
-    __shared__ double s_red[4 * kWavesPerBlock + 4];
+    __shared__ double s_red[4 * kNodesPerItem + 4];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
     __shared__ typename std::conditional<kSingle, NodeCache, int>::type s_nc_store;
     NodeCache* s_nc = kSingle ? reinterpret_cast<NodeCache*>(&s_nc_store) : nullptr;
@@ -1821,7 +1818,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 x[1] = s_lo[0];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    x[2 + q] = ci1 > ci0 ? (unsigned long long)__double_as_longlong(s_red[4 * kWavesPerBlock + q]) : 0ull;
+                    x[2 + q] = ci1 > ci0 ? (unsigned long long)__double_as_longlong(s_red[4 * kNodesPerItem + q]) : 0ull;
                 x[6] = s_bad[0] ? 1ull : 0ull;
                 pack62(x, step_tag(step), w);
                 unsigned long long* aw = reinterpret_cast<unsigned long long*>(&ws->arrive[blockIdx.x * kArriveStride]);
