@@ -1699,7 +1699,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                         return pn;
                     };
                     auto chunks = [&](auto fast) {
-                        for (int c = 0; c < nch; ++c) {
+#pragma unroll
+                        for (int c = 0; c < kChunks; ++c) {
+                            if (c >= nch) break;
                             TilePix pn;
                             if (c + 1 < nch)
                                 pn = tile_pix(Da, ta, c + 1, lane);
@@ -1720,7 +1722,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                         const float qa = 0.25f * (1.f - xi), qb = 0.25f * (1.f + xi);
                         const int rstep = kWave >> gg.sh;
                         int row = gg.row0 + (lane >> gg.sh);
-                        for (int c = 0; c < nch; ++c) {
+                        // unrolled over the chunks: each chunk's prefetched f stays in its
+                        // own registers (no copy at the back edge waiting for the load)
+#pragma unroll
+                        for (int c = 0; c < kChunks; ++c) {
+                            if (c >= nch) break;
                             TilePix pn;
                             if (c + 1 < nch)
                                 pn.f = tile_f(Da, ta, c + 1, lane);
