@@ -246,6 +246,10 @@ __device__ __forceinline__ float2 ld2_fresh(const float* p, int n) {
     return make_float2(__uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
 }
 
+__device__ __forceinline__ double ld_fresh(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Publication without an L2 write-back.  Every store another block reads
 // during the launch is an agent-scope store (st_wt, st2_wt, device atomics),
 // which the L2 writes through to the coherence point and acknowledges from
@@ -944,8 +948,7 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
     const bool writer = blockIdx.x == 0;
     const unsigned long long admit =
         sh ? __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-    // Plain loads: the barrier's acquire fence orders them after every
-    // block's release of the sums.
+    // Coherent loads of the sums other blocks published before their arrival.
     for (int pw = wave; pw < P; pw += kWavesPerBlock) {
         const int p = __builtin_amdgcn_readfirstlane(pw);
         const DecideDesc& D = s_dd[p];
@@ -957,18 +960,23 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
         double sg = 0.0, sn = 0.0, sd = 0.0, sE = 0.0;
         if (had) {
             if (lane < kSlots) {
-                hi = gld(&ws->hi[buf][p][0])[lane];
-                lo = gld(&ws->lo[buf][p][0])[lane];
+                hi = __hip_atomic_load(&ws->hi[buf][p][0] + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                lo = __hip_atomic_load(&ws->lo[buf][p][0] + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (lane == kWave - 1) bad = gld(&ws->bad[buf][p])[0];
+            if (lane == kWave - 1)
+                bad = __hip_atomic_load(&ws->bad[buf][p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const int nch = D.nchunks;
-            const D2IC_GLOBAL d4v* ncp = gld(reinterpret_cast<const d4v*>(D.NCP)) + (size_t)par * nch;
+            const double* ncp = D.NCP + 4 * (size_t)par * nch;
             for (int c0 = 0; c0 < nch; c0 += 4 * kWave) {
                 d4v v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int c = c0 + u * kWave + lane;
-                    v[u] = c < nch ? ncp[c] : d4v{0.0, 0.0, 0.0, 0.0};
+                    v[u] = d4v{0.0, 0.0, 0.0, 0.0};
+                    if (c < nch) {
+                        const double* q = ncp + 4 * (size_t)c;
+                        v[u] = d4v{ld_fresh(q), ld_fresh(q + 1), ld_fresh(q + 2), ld_fresh(q + 3)};
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1890,12 +1898,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                // acquire the step's data for this block's decision; a served
-                // session also releases the descriptors serve_admit writes
-                if constexpr (kServe)
-                    __threadfence();
-                else
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                // a served session releases the descriptors serve_admit writes
+                // and acquires what its new requests' host-written inputs left
+                // in this XCD's L2; a fixed batch reads the step's data with
+                // coherent loads and needs no fence
+                if constexpr (kServe) __threadfence();
                 bool stop = false;
                 if constexpr (kServe)
                     stop = serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
@@ -1932,7 +1939,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // served sessions: inputs of newly admitted requests were written
+            // by the host or other kernels while this kernel ran
+            if constexpr (kServe) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (word & kReleaseAbort) s_ok = 0;
             if (word & kReleaseStop) s_stop = 1;
         }
