@@ -756,6 +756,18 @@ def main() -> int:
     traffic = measured_traffic()
     log(f"[bench] {K} frames in {elapsed:.3f} s -> {fps:.2f} frames/s; iters {iters}; passes {passes}")
 
+    # grid autotuner (CGPlan.autotune) on the headline plan, after the timed
+    # frames: short fixed-pass solves of frame 1 on each candidate grid, the
+    # work-based default (0) included; the headline used the default unless
+    # --autotune was given
+    t_tune = solver.autotune_grid(dic.state, frames[0], cache=False)
+    grid_tune = {"default_ms": t_tune["default_ms"], "tuned_blocks": t_tune["blocks"],
+                 "tuned_ms": t_tune["ms"][t_tune["blocks"]], "ms": {str(k): v for k, v in t_tune["ms"].items()},
+                 "solve": "8 iterations from u = 0, tol 0, median of 3 after one warm-up",
+                 "note": "results are bitwise the same on every grid (tests/test_gpu_grid.py)"}
+    solver.plan.set_grid(tune["blocks"] if tune else 0)
+    log(f"[bench] grid autotune: {grid_tune}")
+
     local = None
     if not args.no_local:
         log("[bench] local refinement leg: 3 Gauss-Newton sweeps from the last frame's solution")
@@ -831,6 +843,7 @@ def main() -> int:
             "mean_solve_ms": float(np.mean(solve_ms)),
         },
         "cpu_baseline": cpu,
+        "grid_autotune": grid_tune,
         "accuracy": accuracy,
         "batch": batch,
         "serving": serving,

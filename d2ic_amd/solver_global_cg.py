@@ -14,9 +14,10 @@ available) or "linear"/"bilinear" (the reference's in-repo bilinear sampler,
 """
 from __future__ import annotations
 
-import dataclasses
-
 import ctypes
+import dataclasses
+import json
+import os
 import sys
 import warnings
 from dataclasses import dataclass
@@ -72,6 +73,42 @@ def _normalize_interpolation(interpolation: str) -> str:
     return value
 
 
+def _autotune_cache_path(cache):
+    """File of the persisted grid-autotune results (None: no cache)."""
+    if cache is False:
+        return None
+    if cache is None:
+        cache = os.environ.get("D2IC_AUTOTUNE_CACHE") or os.path.join(
+            os.path.expanduser("~"), ".cache", "d2ic_amd", "grid_autotune.json")
+    return str(cache)
+
+
+def _autotune_cache_load(path) -> dict:
+    try:
+        with open(path, "r", encoding="utf-8") as fh:
+            data = json.load(fh)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _autotune_cache_store(path, key, entry) -> None:
+    """Add one entry (read-modify-write, replaced atomically; a failure to
+    write leaves the tuning result in effect and is not an error)."""
+    try:
+        data = _autotune_cache_load(path)
+        data[key] = entry
+        d = os.path.dirname(path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w", encoding="utf-8") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
 class CGPlan:
     """Owner of a device plan (``d2ic_cg_plan``) built from PixelAssets."""
 
@@ -125,15 +162,25 @@ class CGPlan:
         self.grid_blocks = int(blocks)
 
     def autotune(self, def_image, alpha_reg: float = 0.1, max_iter: int = 8, candidates=None,
-                 repeats: int = 3) -> dict:
+                 repeats: int = 3, cache=None) -> dict:
         """Choose the persistent grid size by timing short solves.
 
-        Each candidate grid (blocks, multiples of 8 up to one block per CU)
-        runs ``repeats`` solves of ``max_iter`` iterations from u = 0 with
-        tol = 0, so every candidate does the same passes; the kernel time is
-        read from the HIP events around the launch and the fastest median
-        wins and is kept (``set_grid``).  The reference image must be set.
-        Returns ``{"blocks": best, "ms": {blocks: median ms}}``."""
+        Each candidate grid (blocks, multiples of 8 up to one block per CU,
+        and 0 = the work-based default) runs ``repeats`` solves of
+        ``max_iter`` iterations from u = 0 with tol = 0, so every candidate
+        does the same passes; the kernel time is read from the HIP events
+        around the launch and the fastest median wins and is kept
+        (``set_grid``).  The reference image must be set.  The grid is the
+        one launch parameter that leaves results bitwise unchanged (every
+        reduction is an order-independent fixed-point sum); tile shape and
+        node-item size change rounding and are fixed by the element.
+
+        ``cache``: path of a JSON file keyed on (device, interpolation, image
+        shape, pixels, tiles, nodes, candidates, iterations); None uses
+        ``$D2IC_AUTOTUNE_CACHE`` or ``~/.cache/d2ic_amd/grid_autotune.json``,
+        False disables it.  A cached entry is applied without timing.
+        Returns ``{"blocks": best, "ms": {blocks: median ms}, "default_ms":
+        ms of grid 0, "cached": bool}``."""
         t = _lib.torch()
         if self._ref_key is None:
             raise RuntimeError("set_reference() must be called before autotune()")
@@ -141,10 +188,23 @@ class CGPlan:
         if candidates is None:
             cu = int(t.cuda.get_device_properties(_lib.device()).multi_processor_count)
             candidates = sorted({max(8, (cu * k // 8 + 7) // 8 * 8) for k in (2, 4, 6, 8)})
+        cands = sorted({0, *(int(b) for b in candidates)})
+        path = _autotune_cache_path(cache)
+        key = None
+        if path is not None:
+            info = self.info()
+            key = "|".join(str(v) for v in (
+                t.cuda.get_device_name(_lib.device()), self._ref_key[2], "x".join(map(str, self._ref_key[1])),
+                info["n_pixels"], info["n_tiles"], info["n_nodes"], ",".join(map(str, cands)), int(max_iter)))
+            hit = _autotune_cache_load(path).get(key)
+            if hit is not None and int(hit["blocks"]) in cands:
+                self.set_grid(int(hit["blocks"]))
+                ms = {int(k): float(v) for k, v in hit["ms"].items()}
+                return {"blocks": int(hit["blocks"]), "ms": ms, "default_ms": ms.get(0), "cached": True}
         u = t.zeros((self.n_nodes, 2), dtype=t.float32, device=_lib.device())
         cnt = t.zeros((2,), dtype=t.int32, device=_lib.device())
         times = {}
-        for b in candidates:
+        for b in cands:
             self.set_grid(int(b))
             ms = []
             for _ in range(int(repeats) + 1):  # the first run warms caches and is dropped
@@ -156,7 +216,9 @@ class CGPlan:
             times[int(b)] = float(np.median(ms[1:]))
         best = min(times, key=times.get)
         self.set_grid(best)
-        return {"blocks": best, "ms": times}
+        if path is not None:
+            _autotune_cache_store(path, key, {"blocks": best, "ms": {str(k): v for k, v in times.items()}})
+        return {"blocks": best, "ms": times, "default_ms": times[0], "cached": False}
 
     def set_reference(self, ref_image, interpolation: str):
         t = _lib.torch()

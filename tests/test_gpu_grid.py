@@ -14,7 +14,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_solve_is_bitwise_identical_across_grids(gpu):
+def test_solve_is_bitwise_identical_across_grids(gpu, tmp_path):
     from d2ic_amd import DICMeshBased, GlobalCGSolver, MeshDICConfig, mask_to_mesh_assets, speckle
 
     H = W = 512
@@ -41,11 +41,17 @@ def test_solve_is_bitwise_identical_across_grids(gpu):
         np.testing.assert_array_equal(u, u0)
         np.testing.assert_array_equal(h, h0)
 
-    tune = solver.autotune_grid(dic.state, de, candidates=(16, 64, 128), max_iter=6, repeats=2)
+    cache = tmp_path / "grid_autotune.json"
+    tune = solver.autotune_grid(dic.state, de, candidates=(16, 64, 128), max_iter=6, repeats=2, cache=str(cache))
     print(f"autotune {tune}")
-    assert tune["blocks"] in (16, 64, 128) and set(tune["ms"]) == {16, 64, 128}
-    assert all(v > 0.0 for v in tune["ms"].values())
+    assert tune["blocks"] in (0, 16, 64, 128) and set(tune["ms"]) == {0, 16, 64, 128}
+    assert all(v > 0.0 for v in tune["ms"].values()) and tune["default_ms"] == tune["ms"][0]
+    assert not tune["cached"] and cache.exists()
     assert solver.plan.grid_blocks == tune["blocks"]
+    # the persisted choice is applied without timing
+    solver.plan.set_grid(0)
+    again = solver.autotune_grid(dic.state, de, candidates=(16, 64, 128), max_iter=6, repeats=2, cache=str(cache))
+    assert again["cached"] and again["blocks"] == tune["blocks"] and solver.plan.grid_blocks == tune["blocks"]
     dic.set_initial_guess(None)
     res = dic.run(de)
     np.testing.assert_array_equal(res.u_nodal.cpu().numpy(), u0)

@@ -60,3 +60,22 @@ def test_pending_solve_finish_fills_iterations_once():
     bad = PendingSolve(solver, res, None, Counters([-1, 3]), None)
     with pytest.raises(RuntimeError, match="did not finish"):
         bad.finish()
+
+
+def test_grid_autotune_cache_file(tmp_path, monkeypatch):
+    """The persisted grid-autotune cache: default path from the environment,
+    disabled with False, entries added without losing earlier ones, and a
+    corrupt file read as empty."""
+    from d2ic_amd import solver_global_cg as m
+
+    monkeypatch.setenv("D2IC_AUTOTUNE_CACHE", str(tmp_path / "c.json"))
+    assert m._autotune_cache_path(None) == str(tmp_path / "c.json")
+    assert m._autotune_cache_path(False) is None
+    p = str(tmp_path / "sub" / "cache.json")
+    assert m._autotune_cache_load(p) == {}
+    m._autotune_cache_store(p, "a", {"blocks": 64, "ms": {"0": 1.0, "64": 0.5}})
+    m._autotune_cache_store(p, "b", {"blocks": 0, "ms": {"0": 0.2}})
+    d = m._autotune_cache_load(p)
+    assert set(d) == {"a", "b"} and d["a"]["blocks"] == 64
+    (tmp_path / "bad.json").write_text("{not json")
+    assert m._autotune_cache_load(str(tmp_path / "bad.json")) == {}
