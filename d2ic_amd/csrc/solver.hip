@@ -23,8 +23,8 @@
 //   tiles:      every tile evaluates the residual at the trial point (d formed
 //               on the fly from g, d_prev and beta), writes its partial
 //               gradient (8 floats) and adds its sum of r^2;
-//   node items: once the blocks holding the tiles of their nodes have
-//               published those partials (a wait on those blocks only, not a
+//   node items: once the waves holding the tiles of their nodes have
+//               published those partials (a wait on those waves only, not a
 //               grid barrier), node items form d, the trial point, the
 //               gradient there (gather through the node CSR plus the spring
 //               term) and reduce |g|^2, g.(g - g_prev), g.d and the spring
@@ -196,7 +196,7 @@ struct Workspace {
     // single-problem solves, lines 0..7 hold the tagged barrier's release
     // word, one line per XCD)
     alignas(128) unsigned release[kMaxBlocks * kArriveStride];
-    alignas(128) unsigned tiles_done[kMaxBlocks * kArriveStride];  // last step whose tiles a block finished
+    alignas(128) unsigned wave_done[kMaxBlocks * kWavesPerBlock];  // last step whose tiles a (virtual) wave finished
     unsigned long long admit_mask;                 // served mode: slots admitted at the last barrier
     int stop;                                      // served mode: every block leaves after this barrier
     int err;
@@ -1241,12 +1241,11 @@ __device__ __forceinline__ void unpack62(const unsigned long long (&w)[kArrWords
 }
 
 
-// Virtual block holding global tile gt: wave w takes tiles
+// Virtual wave holding global tile gt: wave w takes tiles
 // [nT w / nwaves, nT (w + 1) / nwaves), so gt belongs to
 // w = ceil((gt + 1) nwaves / nT) - 1.
-__device__ __forceinline__ int tile_owner_vb(int gt, int nT, int nwaves) {
-    const int w = (int)(((long long)(gt + 1) * nwaves + nT - 1) / nT) - 1;
-    return w / kWavesPerBlock;
+__device__ __forceinline__ int tile_owner_wave(int gt, int nT, int nwaves) {
+    return (int)(((long long)(gt + 1) * nwaves + nT - 1) / nT) - 1;
 }
 
 // Hardware block of virtual block v (inverse of the XCD-aware numbering).
@@ -1269,7 +1268,7 @@ __device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, i
     const int* tr = reinterpret_cast<const int*>(D->item_tr) + 2 * (size_t)item;
     const int tlo = gld(tr)[0], thi = gld(tr)[1];
     if (thi < tlo || nT <= 0) return true;
-    const int va = tile_owner_vb(toff + tlo, nT, nwaves), vz = tile_owner_vb(toff + thi, nT, nwaves);
+    const int va = tile_owner_wave(toff + tlo, nT, nwaves), vz = tile_owner_wave(toff + thi, nT, nwaves);
     if (threadIdx.x < kWave) {
         const int lane = threadIdx.x;
         const uint64_t tstart = wall_clock64();
@@ -1280,8 +1279,7 @@ __device__ bool wait_tiles(const ProbDesc* __restrict__ D, int item, int toff, i
             for (int v0 = va; v0 <= vz; v0 += kWave) {
                 const int v = v0 + lane;
                 if (v <= vz)
-                    mine = mine && __hip_atomic_load(&ws->tiles_done[block_of_vb(v, nb) * kArriveStride],
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                    mine = mine && __hip_atomic_load(&ws->wave_done[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
                                        (unsigned)(step + 1);
             }
             if (__all(mine)) break;
@@ -1335,7 +1333,7 @@ __device__ __forceinline__ void set_plan1(const D2IC_LDS DecideDesc* s_dd, int f
 // out of line (inlined, it changes the register allocation of the tile
 // pipeline); its LDS and global arguments are typed so that its accesses are
 // LDS and global instructions rather than flat ones.
-__device__ __noinline__ void tagged_master(ProbState* __restrict__ S, D2IC_GLOBAL Workspace* ws, int step, int nb,
+__device__ __noinline__ void tagged_master(D2IC_GLOBAL Workspace* ws, int step, int nb,
                                            int nC, const D2IC_LDS DecideDesc* s_dd, D2IC_LDS ProbState* s_st,
                                            D2IC_LDS int* s_flags, D2IC_LDS float* s_alpha, D2IC_LDS float* s_beta,
                                            D2IC_LDS int* s_hdr, D2IC_LDS unsigned long long* s_hi,
@@ -1440,8 +1438,7 @@ __device__ __noinline__ void tagged_master(ProbState* __restrict__ S, D2IC_GLOBA
             ProbState st = *(const ProbState*)s_st;
             const DecideDesc dd = *(const DecideDesc*)s_dd;
             fl = decide(dd, st, sr2, (fl_bits & 1ull) != 0, sg, sn, sd, sE, true);
-            *(ProbState*)s_st = st;
-            store_state(S, st);
+            *(ProbState*)s_st = st;  // stored to S when the kernel leaves
             alpha = st.alpha;
             beta = st.beta;
         }
@@ -1782,6 +1779,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     id_n2 = id_n3;
                 }
             }
+            // this wave's tile partials are published once its stores have
+            // landed; node items wait for the waves in their tile range.  Every
+            // wave publishes, also one without tiles (a wave inside an item's
+            // range may own none when there are fewer tiles than waves)
+            drain_stores();
+            if (lane == 0)
+                __hip_atomic_store(&ws->wave_done[gwave], (unsigned)(step + 1), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
         if (!tagged) {
@@ -1798,14 +1803,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         //      [ceil(vb nC / nb), ceil((vb + 1) nC / nb)); an item first waits
         //      for the blocks holding the tiles of its nodes ----
         const int ci0 = item_begin(vb, nC, nb), ci1 = item_begin(vb + 1, nC, nb);
-        if (nC > 0) {
-            drain_stores();
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                __hip_atomic_store(&ws->tiles_done[blockIdx.x * kArriveStride], (unsigned)(step + 1),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+
         for (int c = ci0; c < ci1; ++c) {
             const int p = kSingle ? 0 : find_prob(s_coff, P, c);
             const ProbDesc* D = descs + p;
@@ -1847,7 +1845,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         }
         if (tagged) {
             if (blockIdx.x == 0) {
-                tagged_master(S, gst(ws), step, nb, nC, (const D2IC_LDS DecideDesc*)s_dd, (D2IC_LDS ProbState*)s_st,
+                tagged_master(gst(ws), step, nb, nC, (const D2IC_LDS DecideDesc*)s_dd, (D2IC_LDS ProbState*)s_st,
                               (D2IC_LDS int*)s_flags, (D2IC_LDS float*)s_alpha, (D2IC_LDS float*)s_beta,
                               (D2IC_LDS int*)s_hdr, (D2IC_LDS unsigned long long*)s_hi,
                               (D2IC_LDS unsigned long long*)s_lo, (D2IC_LDS int*)s_bad,
@@ -1953,6 +1951,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             block_decide(s_dd, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_hi, s_lo,
                          s_bad);
     }
+    // tagged barrier: block 0 keeps the problem state in LDS only while it runs
+    if (tagged && blockIdx.x == 0 && threadIdx.x == 0) store_state(S, s_st[0]);
     if (sh && blockIdx.x == 0 && threadIdx.x == 0) {
         __threadfence_system();
         __hip_atomic_store(&sh->alive, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
