@@ -2136,16 +2136,50 @@ __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, i
     if (split) atomicOr(any_split, 1);
 }
 
-__global__ void k_group_ntiles(const int32_t* __restrict__ gstart, int64_t ng, int64_t np, int32_t* __restrict__ nt) {
+// Tiles of a group: runs of its pixels in their (row-major) order, closed at
+// kTile pixels or when the run would span more than rmax pixel rows, with
+// rmax chosen so that a tile's staged box -- (columns + 5) x (rows + 5),
+// plus one column and one row for a displacement range straddling an integer
+// -- fits the wave's kBoxMax-float LDS slot for the group's column span w:
+// (w + 6)(rmax + 6) <= kBoxMax.  Without the row limit, 256-pixel runs of
+// partly covered 20-24 px elements (cut by a hole or the ROI edge) span up to
+// the element's full height, their boxes miss the slot, and such a tile
+// sampling from global memory took ~8 us longer than the others, so the
+// whole pixel pass waited for it.  One thread per group: tl[s] = tile within
+// the group * kTile + slot, nt[g] = tiles of the group.
+__global__ void k_group_tiling(const int32_t* __restrict__ gstart, int64_t ng, int64_t np,
+                               const int32_t* __restrict__ perm, const float* __restrict__ coords,
+                               int32_t* __restrict__ tl, int32_t* __restrict__ nt) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
-    int64_t e = (g + 1 < ng) ? gstart[g + 1] : np;
-    nt[g] = (int32_t)((e - gstart[g] + kTile - 1) / kTile);
+    const int64_t s0 = gstart[g], s1 = (g + 1 < ng) ? gstart[g + 1] : np;
+    int cmin = INT32_MAX, cmax = INT32_MIN;
+    for (int64_t s = s0; s < s1; ++s) {
+        const int c = (int)floorf(coords[2 * (int64_t)perm[s]]);
+        cmin = min(cmin, c);
+        cmax = max(cmax, c);
+    }
+    const long long w = (long long)cmax - cmin + 1;
+    const int rmax = (int)max(1LL, (long long)kBoxMax / (w + 6) - 6);
+    int tile = 0, pos = 0, r0 = 0;
+    for (int64_t s = s0; s < s1; ++s) {
+        const int r = (int)floorf(coords[2 * (int64_t)perm[s] + 1]);
+        if (s == s0) {
+            r0 = r;
+        } else if (pos == kTile || r - r0 >= rmax) {
+            ++tile;
+            pos = 0;
+            r0 = r;
+        }
+        tl[s] = tile * kTile + pos;
+        ++pos;
+    }
+    nt[g] = s1 > s0 ? tile + 1 : 0;
 }
 
 __global__ void k_tiles(const int32_t* __restrict__ gstart, const int32_t* __restrict__ tstart, int64_t ng, int64_t np,
-                        const int32_t* __restrict__ perm, const int32_t* __restrict__ pnodes, int32_t* __restrict__ tcnt,
-                        int4* __restrict__ tn) {
+                        const int32_t* __restrict__ perm, const int32_t* __restrict__ pnodes,
+                        const int32_t* __restrict__ tl, int32_t* __restrict__ tcnt, int4* __restrict__ tn) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     int s0 = gstart[g];
@@ -2153,9 +2187,10 @@ __global__ void k_tiles(const int32_t* __restrict__ gstart, const int32_t* __res
     int p0 = perm[s0];
     int4 nodes = make_int4(pnodes[4 * (int64_t)p0], pnodes[4 * (int64_t)p0 + 1], pnodes[4 * (int64_t)p0 + 2],
                            pnodes[4 * (int64_t)p0 + 3]);
-    int t = tstart[g];
-    for (int s = s0; s < s1; s += kTile, ++t) {
-        tcnt[t] = min(kTile, s1 - s);
+    // slots of a tile are consecutive, so the tile's last pixel gives its count
+    for (int s = s0; s < s1; ++s) {
+        const int t = tstart[g] + tl[s] / kTile;
+        tcnt[t] = tl[s] % kTile + 1;
         tn[t] = nodes;
     }
 }
@@ -2280,16 +2315,15 @@ __global__ void k_pad_init(int64_t n, float* __restrict__ px, float* __restrict_
     N4[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// sorted pixel s of group g sits at slot (tstart[g] + q / 64) * 64 + q % 64, q = s - gstart[g]
+// sorted pixel s of group g sits at slot (tstart[g] + tl[s] / kTile) * kTile + tl[s] % kTile
 __global__ void k_soa(const int32_t* __restrict__ perm, const int32_t* __restrict__ gid,
-                      const int32_t* __restrict__ gstart, const int32_t* __restrict__ tstart, int64_t np,
+                      const int32_t* __restrict__ tl, const int32_t* __restrict__ tstart, int64_t np,
                       const float* __restrict__ coords, const float* __restrict__ shapeN, float* __restrict__ px,
                       float* __restrict__ py, float4* __restrict__ N4) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= np) return;
     const int g = gid[s] - 1;
-    const int q = (int)s - gstart[g];
-    const int64_t o = (int64_t)(tstart[g] + q / kTile) * kTile + q % kTile;
+    const int64_t o = (int64_t)(tstart[g] + tl[s] / kTile) * kTile + tl[s] % kTile;
     const int64_t p = perm[s];
     px[o] = coords[2 * p];
     py[o] = coords[2 * p + 1];
@@ -2624,7 +2658,10 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         }
         PH(ntg.alloc(sizeof(int32_t) * (ng + 1), st));
         PH(tstart.alloc(sizeof(int32_t) * (ng + 1), st));
-        k_group_ntiles<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), ng, n_pixels, ntg.as<int32_t>());
+        Tmp tl;
+        PH(tl.alloc(sizeof(int32_t) * n_pixels, st));
+        k_group_tiling<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), ng, n_pixels, perm.as<int32_t>(),
+                                                        pixel_coords, tl.as<int32_t>(), ntg.as<int32_t>());
         PH(hipGetLastError());
         PH(hipMemsetAsync(ntg.as<int32_t>() + ng, 0, sizeof(int32_t), st));
         size_t b4 = 0;
@@ -2648,7 +2685,8 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PA(pl->N4, npad);
         PA(pl->F, npad);
         k_tiles<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), tstart.as<int32_t>(), ng, n_pixels,
-                                                 perm.as<int32_t>(), pixel_nodes, pl->tile_cnt, pl->tile_nodes);
+                                                 perm.as<int32_t>(), pixel_nodes, tl.as<int32_t>(), pl->tile_cnt,
+                                                 pl->tile_nodes);
         PH(hipGetLastError());
 
         // group id of every sorted pixel (1-based) = inclusive scan of the heads
@@ -2660,7 +2698,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         PH(hipcub::DeviceScan::InclusiveSum(tmp4.p, b7, head.as<uint8_t>(), gid.as<int32_t>(), (int)n_pixels, st));
         k_pad_init<<<ceil_div(npad, tb), tb, 0, st>>>(npad, pl->px, pl->py, pl->N4);
         PH(hipGetLastError());
-        k_soa<<<ceil_div(n_pixels, tb), tb, 0, st>>>(perm.as<int32_t>(), gid.as<int32_t>(), gstart.as<int32_t>(),
+        k_soa<<<ceil_div(n_pixels, tb), tb, 0, st>>>(perm.as<int32_t>(), gid.as<int32_t>(), tl.as<int32_t>(),
                                                      tstart.as<int32_t>(), n_pixels, pixel_coords, pixel_shapeN,
                                                      pl->px, pl->py, pl->N4);
         PH(hipGetLastError());
