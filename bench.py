@@ -767,6 +767,21 @@ def main() -> int:
                  "note": "results are bitwise the same on every grid (tests/test_gpu_grid.py)"}
     solver.plan.set_grid(tune["blocks"] if tune else 0)
     log(f"[bench] grid autotune: {grid_tune}")
+    # tile-shape autotuner (GlobalCGSolver.autotune_tiling) on the same mesh:
+    # each candidate tiling is a plan of its own, timed like the grids; the
+    # headline plan is restored afterwards
+    t_tile = solver.autotune_tiling(dic.state, frames[0], cache=False)
+    tiling_tune = {"default_ms": t_tile["default_ms"], "tuned": "{}/{}".format(*t_tile["tiling"]),
+                   "tuned_ms": t_tile["ms"]["{}/{}".format(*t_tile["tiling"])], "ms": t_tile["ms"],
+                   "keys": "tile_px/strip_w (tiles close at tile_px pixels; elements wider than 24 px are cut "
+                           "into strips of at most strip_w columns)",
+                   "solve": "8 iterations from u = 0, tol 0, median of 3 after one warm-up, default grid",
+                   "note": "tilings group the tile partials differently: results agree to rounding "
+                           "(tests/test_gpu_grid.py)"}
+    solver.set_tiling(dic.state)
+    if tune:
+        solver.plan.set_grid(tune["blocks"])
+    log(f"[bench] tiling autotune: {tiling_tune}")
 
     local = None
     if not args.no_local:
@@ -844,6 +859,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "grid_autotune": grid_tune,
+        "tiling_autotune": tiling_tune,
         "accuracy": accuracy,
         "batch": batch,
         "serving": serving,

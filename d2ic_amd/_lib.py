@@ -36,6 +36,8 @@ _SIGS = {
     "d2ic_node_pixel_degree": [_vp, _i64, _i64, _vp, ctypes.POINTER(_i32), _vp],
     "d2ic_node_pixel_table": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
     "d2ic_cg_plan_create": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i32, ctypes.POINTER(_vp), _vp],
+    "d2ic_cg_plan_create_tiled": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i32, _i32, _i32,
+                                  ctypes.POINTER(_vp), _vp],
     "d2ic_cg_plan_destroy": [_vp],
     "d2ic_cg_set_reference": [_vp, _vp, _i32, _i32, _i32, _vp],
     "d2ic_cg_plan_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),

@@ -62,6 +62,9 @@ struct d2ic_cg_plan {
     // persistent grid size for single-problem solves led by this plan (0: the
     // work-based default), set by d2ic_cg_plan_set_grid (the autotuner)
     int grid_blocks = 0;
+    // tiling chosen at creation (d2ic_cg_plan_create_tiled): pixels per tile
+    // at most, and the column-strip width of elements wider than 24 px
+    int tile_px = 256, strip_w = 16;
 };
 
 namespace d2ic {

@@ -2095,9 +2095,10 @@ __global__ void k_heads(const unsigned long long* __restrict__ k, int64_t n, uin
 }
 
 // Elements wider than kStripMinW pixels (and of more than one tile) are
-// regrouped before tiling into ns = ceil(w / 16) column strips of balanced
-// width ceil(w / ns), relative to the element's first pixel column; the
-// 256-pixel runs of a strip then span <= 16 columns, and their staged image
+// regrouped before tiling into ns = ceil(w / strip_w) column strips of
+// balanced width ceil(w / ns), relative to the element's first pixel column
+// (strip_w = 16 by default); the 256-pixel runs of a 16-column strip then
+// span <= 16 columns, and their staged image
 // box ((columns + 5) x (rows + 5) plus the displacement range) fits the
 // wave's 512-float LDS slot.  Row-major runs of a 32-pixel-wide element (8
 // rows x 32 columns, a 37-38 x 13-14 box) miss it on ~6 % of tiles, whose
@@ -2108,14 +2109,14 @@ __global__ void k_heads(const unsigned long long* __restrict__ k, int64_t n, uin
 // current (row-major) order, so a stable sort keeps rows in order.
 constexpr int kStripMinW = 24;
 __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, int64_t np,
-                               const int32_t* __restrict__ perm, const float* __restrict__ coords,
-                               unsigned long long* __restrict__ key2, int* __restrict__ any_split) {
+                               const int32_t* __restrict__ perm, const float* __restrict__ coords, int tile_px,
+                               int strip_w, unsigned long long* __restrict__ key2, int* __restrict__ any_split) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int64_t s0 = gstart[g], s1 = (g + 1 < ng) ? gstart[g + 1] : np;
     int xmin = 0, sw = 1;
     bool split = false;
-    if (s1 - s0 > kTile) {
+    if (s1 - s0 > tile_px) {
         int xmax = INT32_MIN;
         xmin = INT32_MAX;
         for (int64_t s = s0; s < s1; ++s) {
@@ -2125,7 +2126,7 @@ __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, i
         }
         const int w = xmax - xmin + 1;
         split = w > kStripMinW;
-        const int ns = (w + 15) / 16;
+        const int ns = (w + strip_w - 1) / strip_w;
         sw = (w + ns - 1) / ns;
     }
     for (int64_t s = s0; s < s1; ++s) {
@@ -2137,7 +2138,8 @@ __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, i
 }
 
 // Tiles of a group: runs of its pixels in their (row-major) order, closed at
-// kTile pixels or when the run would span more than rmax pixel rows, with
+// tile_px (<= kTile) pixels or when the run would span more than rmax pixel
+// rows, with
 // rmax chosen so that a tile's staged box -- (columns + 5) x (rows + 5),
 // plus one column and one row for a displacement range straddling an integer
 // -- fits the wave's kBoxMax-float LDS slot for the group's column span w:
@@ -2148,7 +2150,7 @@ __global__ void k_group_strips(const int32_t* __restrict__ gstart, int64_t ng, i
 // whole pixel pass waited for it.  One thread per group: tl[s] = tile within
 // the group * kTile + slot, nt[g] = tiles of the group.
 __global__ void k_group_tiling(const int32_t* __restrict__ gstart, int64_t ng, int64_t np,
-                               const int32_t* __restrict__ perm, const float* __restrict__ coords,
+                               const int32_t* __restrict__ perm, const float* __restrict__ coords, int tile_px,
                                int32_t* __restrict__ tl, int32_t* __restrict__ nt) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
@@ -2166,7 +2168,7 @@ __global__ void k_group_tiling(const int32_t* __restrict__ gstart, int64_t ng, i
         const int r = (int)floorf(coords[2 * (int64_t)perm[s] + 1]);
         if (s == s0) {
             r0 = r;
-        } else if (pos == kTile || r - r0 >= rmax) {
+        } else if (pos == tile_px || r - r0 >= rmax) {
             ++tile;
             pos = 0;
             r0 = r;
@@ -2515,8 +2517,20 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
                         const int32_t* pixel_elements, int64_t n_pixels, int64_t n_nodes, const int32_t* nb_index,
                         const int32_t* nb_degree, const float* nb_weight, const float* reg_weight,
                         int32_t nb_max_degree, d2ic_cg_plan** plan_out, void* stream) {
+    return d2ic_cg_plan_create_tiled(pixel_coords, pixel_nodes, pixel_shapeN, pixel_elements, n_pixels, n_nodes,
+                                     nb_index, nb_degree, nb_weight, reg_weight, nb_max_degree, kTile, 16, plan_out,
+                                     stream);
+}
+
+int d2ic_cg_plan_create_tiled(const float* pixel_coords, const int32_t* pixel_nodes, const float* pixel_shapeN,
+                              const int32_t* pixel_elements, int64_t n_pixels, int64_t n_nodes,
+                              const int32_t* nb_index, const int32_t* nb_degree, const float* nb_weight,
+                              const float* reg_weight, int32_t nb_max_degree, int32_t tile_px, int32_t strip_w,
+                              d2ic_cg_plan** plan_out, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     D2IC_CHECK_ARG(plan_out != nullptr, "plan must not be NULL");
+    D2IC_CHECK_ARG(tile_px >= kWave && tile_px <= kTile, "tile_px must lie in [64, 256]");
+    D2IC_CHECK_ARG(strip_w >= 4 && strip_w <= 24, "strip_w must lie in [4, 24]");
     D2IC_CHECK_ARG(n_pixels >= 0 && n_nodes >= 0 && nb_max_degree >= 0, "sizes must be non-negative");
     D2IC_CHECK_ARG(n_nodes < (1 << 21), "at most 2^21 - 1 nodes are supported");
     D2IC_CHECK_ARG(4 * n_pixels < (int64_t)INT32_MAX, "too many pixels");
@@ -2547,6 +2561,8 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         }                                                                          \
     } while (0)
     PH(hipGetDevice(&pl->device));
+    pl->tile_px = tile_px;
+    pl->strip_w = strip_w;
     pl->np = n_pixels;
     pl->nn = n_nodes;
     pl->nb_maxdeg = nb_max_degree;
@@ -2636,8 +2652,8 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
             PH(any.alloc(sizeof(int), st));
             PH(hipMemsetAsync(any.p, 0, sizeof(int), st));
             k_group_strips<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), ng, n_pixels, perm.as<int32_t>(),
-                                                            pixel_coords, keys.as<unsigned long long>(),
-                                                            any.as<int>());
+                                                            pixel_coords, tile_px, strip_w,
+                                                            keys.as<unsigned long long>(), any.as<int>());
             PH(hipGetLastError());
             int hany = 0;
             PH(hipMemcpyAsync(&hany, any.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -2661,7 +2677,7 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
         Tmp tl;
         PH(tl.alloc(sizeof(int32_t) * n_pixels, st));
         k_group_tiling<<<ceil_div(ng, tb), tb, 0, st>>>(gstart.as<int32_t>(), ng, n_pixels, perm.as<int32_t>(),
-                                                        pixel_coords, tl.as<int32_t>(), ntg.as<int32_t>());
+                                                        pixel_coords, tile_px, tl.as<int32_t>(), ntg.as<int32_t>());
         PH(hipGetLastError());
         PH(hipMemsetAsync(ntg.as<int32_t>() + ng, 0, sizeof(int32_t), st));
         size_t b4 = 0;

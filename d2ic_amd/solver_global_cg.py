@@ -110,9 +110,16 @@ def _autotune_cache_store(path, key, entry) -> None:
 
 
 class CGPlan:
-    """Owner of a device plan (``d2ic_cg_plan``) built from PixelAssets."""
+    """Owner of a device plan (``d2ic_cg_plan``) built from PixelAssets.
 
-    def __init__(self, pix: PixelAssets):
+    ``tile_px`` (64..256) and ``strip_w`` (4..24) choose the tiling: tiles
+    close at ``tile_px`` pixels, and elements wider than 24 px are cut into
+    column strips of at most ``strip_w`` pixels.  The defaults (256, 16) are
+    the plan ``d2ic_cg_plan_create`` builds."""
+
+    DEFAULT_TILING = (256, 16)
+
+    def __init__(self, pix: PixelAssets, tile_px: int = 256, strip_w: int = 16):
         t = _lib.require_gpu()
         self._keep = []
 
@@ -135,9 +142,10 @@ class CGPlan:
         self.node_active = (dev(pix.node_pixel_degree, t.int32) > 0) | (nb_deg > 0)
         md = int(nb_idx.shape[1]) if nb_idx.dim() == 2 else 0
         h = ctypes.c_void_p(0)
-        _lib.call("d2ic_cg_plan_create", _lib.ptr(coords), _lib.ptr(pnodes), _lib.ptr(shapeN), _lib.ptr(pelts),
+        self.tiling = (int(tile_px), int(strip_w))
+        _lib.call("d2ic_cg_plan_create_tiled", _lib.ptr(coords), _lib.ptr(pnodes), _lib.ptr(shapeN), _lib.ptr(pelts),
                   self.n_pixels, self.n_nodes, _lib.ptr(nb_idx), _lib.ptr(nb_deg), _lib.ptr(nb_w), _lib.ptr(reg_w),
-                  md, ctypes.byref(h), _lib.stream())
+                  md, self.tiling[0], self.tiling[1], ctypes.byref(h), _lib.stream())
         self.handle = h
         self._ref_key = None
         self.grid_blocks = 0
@@ -172,11 +180,12 @@ class CGPlan:
         around the launch and the fastest median wins and is kept
         (``set_grid``).  The reference image must be set.  The grid is the
         one launch parameter that leaves results bitwise unchanged (every
-        reduction is an order-independent fixed-point sum); tile shape and
-        node-item size change rounding and are fixed by the element.
+        reduction is an order-independent fixed-point sum); the tiling
+        changes rounding and is chosen with the plan
+        (``GlobalCGSolver.autotune_tiling``).
 
         ``cache``: path of a JSON file keyed on (device, interpolation, image
-        shape, pixels, tiles, nodes, candidates, iterations); None uses
+        shape, pixels, tiling, tiles, nodes, candidates, iterations); None uses
         ``$D2IC_AUTOTUNE_CACHE`` or ``~/.cache/d2ic_amd/grid_autotune.json``,
         False disables it.  A cached entry is applied without timing.
         Returns ``{"blocks": best, "ms": {blocks: median ms}, "default_ms":
@@ -195,30 +204,43 @@ class CGPlan:
             info = self.info()
             key = "|".join(str(v) for v in (
                 t.cuda.get_device_name(_lib.device()), self._ref_key[2], "x".join(map(str, self._ref_key[1])),
-                info["n_pixels"], info["n_tiles"], info["n_nodes"], ",".join(map(str, cands)), int(max_iter)))
+                info["n_pixels"], "{}/{}".format(*self.tiling), info["n_tiles"], info["n_nodes"],
+                ",".join(map(str, cands)), int(max_iter)))
             hit = _autotune_cache_load(path).get(key)
             if hit is not None and int(hit["blocks"]) in cands:
                 self.set_grid(int(hit["blocks"]))
                 ms = {int(k): float(v) for k, v in hit["ms"].items()}
                 return {"blocks": int(hit["blocks"]), "ms": ms, "default_ms": ms.get(0), "cached": True}
-        u = t.zeros((self.n_nodes, 2), dtype=t.float32, device=_lib.device())
-        cnt = t.zeros((2,), dtype=t.int32, device=_lib.device())
         times = {}
         for b in cands:
             self.set_grid(int(b))
-            ms = []
-            for _ in range(int(repeats) + 1):  # the first run warms caches and is dropped
-                u.zero_()
-                _lib.call("d2ic_cg_solve", self.handle, _lib.ptr(g), int(g.shape[0]), int(g.shape[1]), _lib.ptr(u),
-                          float(alpha_reg), int(max_iter), 0.0, None, _lib.ptr(cnt), _lib.ptr(cnt) + 4,
-                          _lib.stream())
-                ms.append(self.kernel_ms())
-            times[int(b)] = float(np.median(ms[1:]))
+            times[int(b)] = self.time_solves(g, alpha_reg, max_iter, repeats)
         best = min(times, key=times.get)
         self.set_grid(best)
         if path is not None:
             _autotune_cache_store(path, key, {"blocks": best, "ms": {str(k): v for k, v in times.items()}})
         return {"blocks": best, "ms": times, "default_ms": times[0], "cached": False}
+
+    def time_solves(self, def_image, alpha_reg: float = 0.1, max_iter: int = 8, repeats: int = 3,
+                    u0=None) -> float:
+        """Median kernel time (ms) of ``repeats`` solves of ``max_iter``
+        iterations with tol = 0 from ``u0`` (default zeros), after one
+        dropped warm-up solve.  The reference image must be set."""
+        t = _lib.torch()
+        g = _lib.to_dev(def_image, t.float32)
+        start = (t.zeros((self.n_nodes, 2), dtype=t.float32, device=_lib.device()) if u0 is None
+                 else _lib.to_dev(u0, t.float32, (self.n_nodes, 2)))
+        u = t.empty_like(start)
+        cnt = t.zeros((2,), dtype=t.int32, device=_lib.device())
+        ms = []
+        for _ in range(int(repeats) + 1):
+            u.copy_(start)
+            _lib.call("d2ic_cg_solve", self.handle, _lib.ptr(g), int(g.shape[0]), int(g.shape[1]), _lib.ptr(u),
+                      float(alpha_reg), int(max_iter), 0.0, None, _lib.ptr(cnt), _lib.ptr(cnt) + 4, _lib.stream())
+            ms.append(self.kernel_ms())
+        if int(cnt[0].item()) < 0:
+            raise RuntimeError("d2ic_cg_solve did not finish (a grid barrier timed out)")
+        return float(np.median(ms[1:]))
 
     def set_reference(self, ref_image, interpolation: str):
         t = _lib.torch()
@@ -380,6 +402,97 @@ class GlobalCGSolver(SolverBase):
         self._plan.set_reference(state.ref_image, self._interpolation)
         kwargs.setdefault("alpha_reg", float(state.config.reg_strength))
         return self._plan.autotune(def_image, **kwargs)
+
+    def set_tiling(self, state: Any, tile_px: int = 256, strip_w: int = 16) -> None:
+        """Rebuild the plan with another tiling (``CGPlan``); (256, 16) is the
+        default.  The persistent grid returns to the work-based default."""
+        if not self._compiled or self._plan is None:
+            raise RuntimeError("GlobalCGSolver.compile() must be called before set_tiling().")
+        if (int(tile_px), int(strip_w)) != self._plan.tiling:
+            self._replace_plan(CGPlan(state.assets.pixel_data, int(tile_px), int(strip_w)), state)
+
+    def autotune_tiling(self, state: Any, def_image: Array, candidates=None, max_iter: int = 8, repeats: int = 3,
+                        alpha_reg: float | None = None, cache=None) -> dict:
+        """Pick the plan's tiling (``CGPlan`` ``tile_px``, ``strip_w``) for
+        this mesh from short timed solves on ``def_image``: each candidate
+        plan is built and runs ``repeats`` solves of ``max_iter`` iterations
+        (tol 0, from u = 0, default grid); the fastest is kept as the solver's
+        plan, the others are freed.  Candidates default to tiles of 256, 192
+        and 128 pixels with 16-px strips and 256-px tiles with 12- and 8-px
+        strips; the default tiling (256, 16) is always timed.  Tilings differ
+        in how tile partials are grouped, so the kept plan's results differ
+        from the default plan's by rounding only.  A grid chosen before is
+        reset (tune the grid after the tiling: ``autotune``).
+
+        ``cache``: as for the grid autotuner (same file; keyed on device,
+        interpolation, image shape, pixels, nodes, candidates, iterations).
+        Returns ``{"tiling": (tile_px, strip_w), "ms": {"tile_px/strip_w":
+        median ms}, "default_ms": ms of (256, 16), "cached": bool}``.  No
+        reference counterpart: launch tuning only."""
+        if not self._compiled or self._plan is None:
+            raise RuntimeError("GlobalCGSolver.compile() must be called before autotune_tiling().")
+        t = _lib.torch()
+        pix = state.assets.pixel_data
+        if alpha_reg is None:
+            alpha_reg = float(state.config.reg_strength)
+        default = CGPlan.DEFAULT_TILING
+        if candidates is None:
+            candidates = [(256, 16), (192, 16), (128, 16), (256, 12), (256, 8)]
+        cands = [default] + [tuple(int(v) for v in c) for c in candidates if tuple(int(v) for v in c) != default]
+        name = lambda c: f"{c[0]}/{c[1]}"  # noqa: E731
+        ref = self._plan.set_reference(state.ref_image, self._interpolation)
+        g = _lib.to_dev(def_image, t.float32)
+        path = _autotune_cache_path(cache)
+        key = None
+        if path is not None:
+            key = "|".join(str(v) for v in (
+                "tiling", t.cuda.get_device_name(_lib.device()), self._interpolation,
+                "x".join(map(str, ref.shape)), self._plan.n_pixels, self._plan.n_nodes,
+                ",".join(map(name, cands)), int(max_iter)))
+            hit = _autotune_cache_load(path).get(key)
+            if hit is not None and tuple(hit["tiling"]) in cands:
+                best = tuple(int(v) for v in hit["tiling"])
+                if best != self._plan.tiling:
+                    self._replace_plan(CGPlan(pix, *best), state)
+                ms = {str(k): float(v) for k, v in hit["ms"].items()}
+                return {"tiling": best, "ms": ms, "default_ms": ms.get(name(default)), "cached": True}
+        plans = {self._plan.tiling: self._plan}
+        times = {}
+        for c in cands:
+            if c not in plans:
+                plans[c] = CGPlan(pix, *c)
+                plans[c].set_reference(state.ref_image, self._interpolation)
+            plans[c].set_grid(0)
+            times[c] = plans[c].time_solves(g, alpha_reg, max_iter, repeats)
+        best = min(times, key=times.get)
+        keep = plans.pop(best)
+        for p in plans.values():
+            if p is not self._plan:
+                p.close()
+        if keep is not self._plan:
+            self._replace_plan(keep, state)
+        else:
+            keep.set_grid(0)
+        ms = {name(c): v for c, v in times.items()}
+        if path is not None:
+            _autotune_cache_store(path, key, {"tiling": list(best), "ms": ms})
+        return {"tiling": best, "ms": ms, "default_ms": times[default], "cached": False}
+
+    def autotune(self, state: Any, def_image: Array, tiling_candidates=None, grid_candidates=None,
+                 **kwargs) -> dict:
+        """Tiling, then the persistent grid on the kept plan
+        (``autotune_tiling``, ``autotune_grid``); the other keyword arguments
+        (max_iter, repeats, alpha_reg, cache) go to both."""
+        tiling = self.autotune_tiling(state, def_image, candidates=tiling_candidates, **kwargs)
+        grid = self.autotune_grid(state, def_image, candidates=grid_candidates, **kwargs)
+        return {"tiling": tiling, "grid": grid}
+
+    def _replace_plan(self, plan: CGPlan, state: Any) -> None:
+        old = self._plan
+        self._plan = plan
+        plan.set_reference(state.ref_image, self._interpolation)
+        if old is not None and old is not plan:
+            old.close()
 
     def close(self) -> None:
         """Release the device plan."""

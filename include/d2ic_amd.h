@@ -78,7 +78,8 @@ typedef struct d2ic_cg_plan d2ic_cg_plan;
 
 /* Build the solver's internal layout from PixelAssets arrays: pixels grouped
  * by element into tiles of <= 256 pixels (structure-of-arrays; elements wider
- * than 24 pixels by column strips of <= 16 pixels), the
+ * than 24 pixels by column strips of <= 16 pixels; a tile spans few enough
+ * pixel rows for its image box to stage in LDS), the
  * node->tile gather table, and the spring-regularisation stencil
  * (node_neighbor_* and node_reg_weight, solver_global_cg.py:261-286).
  * pixel_elements may be NULL (elements are then identified by their node tuple). */
@@ -86,6 +87,18 @@ int d2ic_cg_plan_create(const float* pixel_coords, const int32_t* pixel_nodes, c
                         const int32_t* pixel_elements, int64_t n_pixels, int64_t n_nodes,
                         const int32_t* nb_index, const int32_t* nb_degree, const float* nb_weight,
                         const float* reg_weight, int32_t nb_max_degree, d2ic_cg_plan** plan, void* stream);
+
+/* The same plan with a chosen tiling (no reference counterpart; what the
+ * tile-shape autotuner varies): tiles close at tile_px pixels (64..256; 256
+ * in d2ic_cg_plan_create) and elements wider than 24 pixels are cut into
+ * column strips of at most strip_w pixels (4..24; 16 in d2ic_cg_plan_create).
+ * Every tiling solves the same problem; the tile partials are summed in other
+ * groupings, so results differ from the default tiling by rounding only. */
+int d2ic_cg_plan_create_tiled(const float* pixel_coords, const int32_t* pixel_nodes, const float* pixel_shapeN,
+                              const int32_t* pixel_elements, int64_t n_pixels, int64_t n_nodes,
+                              const int32_t* nb_index, const int32_t* nb_degree, const float* nb_weight,
+                              const float* reg_weight, int32_t nb_max_degree, int32_t tile_px, int32_t strip_w,
+                              d2ic_cg_plan** plan, void* stream);
 int d2ic_cg_plan_destroy(d2ic_cg_plan* plan);
 
 /* Sample the reference image f at every pixel once (the reference resamples
