@@ -38,8 +38,9 @@
 // alpha = 1 is accepted (the reference's usual case).
 //
 // Reductions are deterministic and independent of the batch composition:
-// per-tile sums of r^2 are converted exactly to a two-word fixed-point integer
-// (resolution 2^-40), pre-summed per block in LDS and added with integer
+// each lane's r^2 of a tile is converted exactly to a two-word fixed-point
+// integer (resolution 2^-40), summed over the wave's tiles, the wave, the
+// block in LDS and added with integer
 // atomics spread over 16 slots; node-item partials are stored per item and
 // summed in a fixed order.  Dot products and the objective are accumulated in
 // FP64 over FP32 arithmetic.
@@ -1636,8 +1637,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
 
         // ---- tiles: contiguous share per wave, software-pipelined: record 2
         //      tiles ahead, staged box 1 tile ahead, pixels 1 chunk ahead ----
-        const int my_t0 = (int)(((long long)nT * vb * kWavesPerBlock) / nwaves);
-        const int my_t1 = (int)(((long long)nT * (vb + 1) * kWavesPerBlock) / nwaves);
         if (nT > 0) {
             const int t0 = (int)(((long long)nT * gwave) / nwaves);
             const int t1 = (int)(((long long)nT * (gwave + 1)) / nwaves);
@@ -1772,18 +1771,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                         chunks(std::false_type{});
                     }
                     reduce8_store(acc, lane, gst(Da->TP + 8 * (size_t)ta));
-                    r2 = wave_sum63(r2);
-                    if (lane == kWave - 1) {
-                        if (!(r2 <= 1e300))
-                            bad = 1;
-                        else
-                            fix_split(r2, hi, lo);
-                    }
+                    // each lane's r^2 of this tile into its fixed-point words; the
+                    // wave adds them (integers: any order) only when the problem
+                    // changes or its tiles end
+                    if (!(r2 <= 1e300))
+                        bad = 1;
+                    else
+                        fix_split(r2, hi, lo);
                     if (pb != pa || it + 1 >= t1) {
+                        const unsigned long long hs = wave_sum63(hi), ls = wave_sum63(lo);
+                        const bool wbad = __any(bad != 0);
                         if (lane == kWave - 1) {
-                            atomicAdd(&s_hi[pa], hi);
-                            atomicAdd(&s_lo[pa], lo);
-                            if (bad) atomicOr(&s_bad[pa], 1);
+                            atomicAdd(&s_hi[pa], hs);
+                            atomicAdd(&s_lo[pa], ls);
+                            if (wbad) atomicOr(&s_bad[pa], 1);
                         }
                         hi = lo = 0ull;
                         bad = 0;
