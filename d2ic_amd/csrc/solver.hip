@@ -1136,7 +1136,7 @@ __device__ bool serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
         }
         if (take) __hip_atomic_store(&sh->admitted_gen[lane], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         if (!pending) seen_bell = bell;  // rescan later while a request waits for its slot
-        __threadfence();
+        if (mask) __threadfence();  // release the copied descriptors (plain stores)
     }
     const uint64_t now = wall_clock64();
     if (any_active || mask || pending) idle_t0 = now;
@@ -1147,16 +1147,22 @@ __device__ bool serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
         __hip_atomic_store(&ws->admit_mask, mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ws->stop, stop ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __threadfence();
+    // the mask and stop words (written through) land before the release word
+    drain_stores();
     return stop;
 }
 
 // Served mode, every block after the barrier: copy the descriptors of the
-// slots admitted at this barrier into the block's LDS tables.
+// slots admitted at this barrier into the block's LDS tables.  An admission
+// also acquires: the new requests' inputs were written by the host or other
+// kernels while this kernel ran, and this XCD's L2 may hold older lines of
+// their buffers (steps without an admission need no fence: the step's own
+// data is read with coherent loads).
 __device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, DecideDesc* s_dd, Workspace* ws) {
     constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
     const unsigned long long admit = __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (admit) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         for (int i = threadIdx.x; i < kMaxSlots * kWords; i += kBlock) {
             const int slot = i / kWords;
             if ((admit >> slot) & 1ull)
@@ -1919,11 +1925,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                // a served session releases the descriptors serve_admit writes
-                // and acquires what its new requests' host-written inputs left
-                // in this XCD's L2; a fixed batch reads the step's data with
-                // coherent loads and needs no fence
-                if constexpr (kServe) __threadfence();
+                // the step's data is read with coherent loads (no fence here);
+                // a served session fences only at an admission (serve_admit,
+                // serve_refresh)
                 bool stop = false;
                 if constexpr (kServe)
                     stop = serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
@@ -1960,9 +1964,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            // served sessions: inputs of newly admitted requests were written
-            // by the host or other kernels while this kernel ran
-            if constexpr (kServe) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (word & kReleaseAbort) s_ok = 0;
             if (word & kReleaseStop) s_stop = 1;
         }
