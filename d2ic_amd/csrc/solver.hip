@@ -1581,7 +1581,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     // XCD-aware numbering: consecutive blocks sit on different XCDs, so give each
     // XCD one contiguous share of the tiles (and of the image).
     const int vb = vb_of((int)blockIdx.x, nb);
-    const int gwave = __builtin_amdgcn_readfirstlane(vb * kWavesPerBlock + wave);
+    // tile-range index of this wave: the 16 waves of a block sit on the CU's 4
+    // SIMDs round-robin (wave w on SIMD w % 4), and the ranges one or two
+    // tiles shorter than the others fall every ~4th index, so consecutive
+    // indices go to waves of one SIMD: each SIMD then gets about one short
+    // range (0.6-1.4 % per frame at 12-20 px elements)
+    static_assert(kWavesPerBlock == 16, "range permutation written for 16 waves");
+    const int gwave = __builtin_amdgcn_readfirstlane(vb * kWavesPerBlock + (wave & 3) * 4 + (wave >> 2));
     const int nwaves = nb * kWavesPerBlock;
 
     const ProbDesc* descs = gdesc;
