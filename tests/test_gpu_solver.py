@@ -80,6 +80,29 @@ def test_cg_first_iterations_track_oracle(gpu, interp):
     np.testing.assert_allclose(ug, u64, atol=2e-5)
 
 
+@pytest.mark.parametrize("e", [20, 24])
+def test_cg_partly_covered_elements_track_oracle(gpu, e):
+    """20 and 24 px elements cut by a hole: their pixel runs span up to the
+    element's full height, so the plan closes tiles at a row limit (the staged
+    image box must fit the LDS slot).  Same first iterates as the FP64 oracle
+    (alpha sequence exact, J rtol 2e-5, u within 2e-5 px)."""
+    print(f"[case] cg partly covered elements e={e}", flush=True)
+    m, mesh, ref, de = structured_case(H=176, W=176, side=160, e=e, seed=4, hole=37.0)
+    dic, res = _solve_gpu(mesh, ref, de, m, "cubic", max_iter=8, tol=1e-9, reg=0.1)
+    oa = O.pixel_assets(mesh.nodes_xy, mesh.elements, ref.shape, roi_mask=m)
+    counts = np.bincount(oa["pixel_elts"], minlength=mesh.elements.shape[0])
+    assert np.any((counts > 0) & (counts < e * e)), "the case must hold partly covered elements"
+    u0 = np.zeros((mesh.nodes_xy.shape[0], 2), np.float32)
+    u64, h64, n64 = O.cg_solve(u0, ref, de, oa, alpha_reg=0.1, max_iter=8, tol=1e-9, interp="cubic",
+                               dtype=np.float64)
+    hg = res.history.cpu().numpy()
+    assert res.diagnostics.info["n_iters"] == n64 == 8
+    np.testing.assert_array_equal(hg[:, 2], h64[:, 2].astype(np.float32))
+    np.testing.assert_allclose(hg[:, 0], h64[:, 0], rtol=2e-5)
+    ug = res.u_nodal.cpu().numpy()
+    np.testing.assert_allclose(ug, u64, atol=2e-5)
+
+
 def test_cg_recovers_translation(gpu):
     print("[case] cg rigid translation", flush=True)
     m, mesh, ref, de = structured_case(H=128, W=128, side=96, e=16, ux=0.37, uy=-0.52)
