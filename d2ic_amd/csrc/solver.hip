@@ -161,7 +161,8 @@ constexpr int kArriveStride = 32;  // one 128-byte line per block's arrival / re
 // release word: step + 1 (mod 2^30) in the low bits, abort and session-stop flags above
 constexpr unsigned kReleaseAbort = 0x80000000u;
 constexpr unsigned kReleaseStop = 0x40000000u;
-constexpr unsigned kReleaseStepMask = 0x3fffffffu;
+constexpr unsigned kReleaseAdmit = 0x20000000u;  // served mode: requests admitted at this barrier
+constexpr unsigned kReleaseStepMask = 0x1fffffffu;
 
 // Mailbox of a served session, in pinned host memory (see d2ic_serve_*).
 // The host fills desc[i], then raises submit_gen[i]; the kernel admits the
@@ -943,12 +944,13 @@ constexpr int kServeBadStart = -3;   // the objective at the initial guess is no
 __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restrict__ S, Workspace* ws, ServeHost* sh,
                              int step, ProbState* s_st, int* s_flags, float* s_alpha, float* s_beta, int* s_toff,
                              int* s_coff, int* s_hdr, unsigned long long* s_hi, unsigned long long* s_lo,
-                             int* s_bad) {
+                             int* s_bad, bool admitted) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int par = step & 1, buf = step % 3;
     const bool writer = blockIdx.x == 0;
+    // the admission mask is read only when the release word says there is one
     const unsigned long long admit =
-        sh ? __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        sh && admitted ? __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
     // Coherent loads of the sums other blocks published before their arrival.
     for (int pw = wave; pw < P; pw += kWavesPerBlock) {
         const int p = __builtin_amdgcn_readfirstlane(pw);
@@ -1100,7 +1102,7 @@ __device__ void block_decide(const DecideDesc* s_dd, int P, ProbState* __restric
 // (identical) block states.
 __device__ bool serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh, const ProbState* s_st,
                             bool any_active, unsigned long long bell, int lane, unsigned long long& seen_bell,
-                            uint64_t& idle_t0) {
+                            uint64_t& idle_t0, bool& admitted) {
     unsigned long long mask = 0ull;
     bool pending = false;
     if (bell != seen_bell) {
@@ -1149,6 +1151,7 @@ __device__ bool serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
     }
     // the mask and stop words (written through) land before the release word
     drain_stores();
+    admitted = mask != 0ull;
     return stop;
 }
 
@@ -1158,8 +1161,10 @@ __device__ bool serve_admit(ProbDesc* descs, int P, Workspace* ws, ServeHost* sh
 // kernels while this kernel ran, and this XCD's L2 may hold older lines of
 // their buffers (steps without an admission need no fence: the step's own
 // data is read with coherent loads).
-__device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, DecideDesc* s_dd, Workspace* ws) {
+__device__ void serve_refresh(const ProbDesc* gdesc, ProbDesc* s_desc, DecideDesc* s_dd, Workspace* ws,
+                              bool admitted) {
     constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
+    if (!admitted) return;
     const unsigned long long admit = __hip_atomic_load(&ws->admit_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (admit) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1569,6 +1574,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     __shared__ int s_hdr[3];
     __shared__ int s_ok;
     __shared__ int s_stop;
+    __shared__ int s_admit;  // served mode: the last release admitted requests
     __shared__ double s_red[4 * kNodesPerItem + 4];
     __shared__ float s_box[kWavesPerBlock][kBoxMax];
     __shared__ typename std::conditional<kSingle, NodeCache, int>::type s_nc_store;
@@ -1619,6 +1625,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
         s_hdr[1] = ws->n_chunks_active;
         s_hdr[2] = ws->any_active;
         s_ok = 1;
+        s_admit = 0;
     }
     // single problem: the block's node item is the same in every step that has
     // node work (items [ceil(vb nC / nb), ...) with nC = nchunks); cache its
@@ -1934,17 +1941,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
                 // the step's data is read with coherent loads (no fence here);
                 // a served session fences only at an admission (serve_admit,
                 // serve_refresh)
-                bool stop = false;
+                bool stop = false, admitted = false;
                 if constexpr (kServe)
-                    stop = serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0);
-                const unsigned word =
-                    ((unsigned)(step + 1) & kReleaseStepMask) | (ok ? 0u : kReleaseAbort) | (stop ? kReleaseStop : 0u);
+                    stop = serve_admit(gdesc, P, ws, sh, s_st, s_hdr[2] != 0, bell, lane, seen_bell, idle_t0,
+                                       admitted);
+                const unsigned word = ((unsigned)(step + 1) & kReleaseStepMask) | (ok ? 0u : kReleaseAbort) |
+                                      (stop ? kReleaseStop : 0u) | (admitted ? kReleaseAdmit : 0u);
                 for (int b = lane; b < nb; b += kWave)
                     __hip_atomic_store(&ws->release[b * kArriveStride], word, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 if (lane == 0) {
                     if (!ok) s_ok = 0;
                     if (stop) s_stop = 1;
+                    s_admit = admitted ? 1 : 0;
                 }
             }
         } else if (threadIdx.x == 0) {
@@ -1972,14 +1981,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
             }
             if (word & kReleaseAbort) s_ok = 0;
             if (word & kReleaseStop) s_stop = 1;
+            s_admit = (word & kReleaseAdmit) ? 1 : 0;
         }
         __syncthreads();
         if (!s_ok) break;
         if (s_stop) break;
-        if constexpr (kServe) serve_refresh(gdesc, s_desc, s_dd, ws);
+        if constexpr (kServe) serve_refresh(gdesc, s_desc, s_dd, ws, s_admit != 0);
         if (!tagged)
             block_decide(s_dd, P, S, ws, sh, step, s_st, s_flags, s_alpha, s_beta, s_toff, s_coff, s_hdr, s_hi, s_lo,
-                         s_bad);
+                         s_bad, s_admit != 0);
     }
     // tagged barrier: block 0 keeps the problem state in LDS only while it runs
     if (tagged && blockIdx.x == 0 && threadIdx.x == 0) store_state(S, s_st[0]);
