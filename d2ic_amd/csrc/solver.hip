@@ -1553,10 +1553,11 @@ using DescTable = typename std::conditional<kServe, ProbDesc*, const ProbDesc* _
 // kSingle: one problem (P == 1), so every problem lookup folds to 0 and the
 // tile pipeline carries no per-problem state.  kInterp: 0 cubic, 1 linear for
 // every problem of the launch (the sampler is compiled in), -1 read per problem.
-template <bool kServe, bool kSingle = false, int kInterp = -1>
+template <bool kServe, bool kSingle = false, int kInterp = -1, bool kTagged = false>
 __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdesc, int P, ProbState* __restrict__ S,
                                                           Workspace* __restrict__ ws, int max_steps, ServeHost* sh) {
     static_assert(!(kServe && kSingle), "a served session has many slots");
+    static_assert(!kTagged || kSingle, "the tagged barrier serves a single problem");
     if constexpr (kSingle) P = 1;
     if constexpr (!kServe) sh = nullptr;  // fixed batches have no mailbox: fold the served paths away
     // served sessions read descriptors from an LDS copy refreshed on admission
@@ -1597,9 +1598,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_persistent(DescTable<kServe> gdes
     const int nwaves = nb * kWavesPerBlock;
 
     const ProbDesc* descs = gdesc;
-    // tagged barrier: single problem, at most one node item per block and at
-    // most four blocks per lane of the master wave
-    const bool tagged = kSingle && nb <= 4 * kWave && gdesc[0].nchunks <= nb;
+    // tagged barrier (compiled in, kTagged): single problem, at most one node
+    // item per block and at most four blocks per lane of the master wave; the
+    // host launches a tagged instance only when that holds
+    constexpr bool tagged = kTagged;
+    if constexpr (kTagged) {
+        if (!(nb <= 4 * kWave && gdesc[0].nchunks <= nb)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                atomicExch(&ws->err, 1);
+                printf("d2ic: tagged barrier launched with %d blocks for %d node items\n", nb, gdesc[0].nchunks);
+            }
+            return;
+        }
+    }
     if constexpr (kServe) {
         constexpr int kWords = (int)(sizeof(ProbDesc) / 8);
         for (int i = threadIdx.x; i < P * kWords; i += kBlock)
@@ -2472,12 +2483,13 @@ thread_local Staging g_stage;
 // Co-resident capacity of a persistent-kernel instance (cached per device).
 int occupancy_blocks(const void* kernel, int* out) {
     static std::mutex mu;
-    constexpr int kKernels = 5;
+    constexpr int kKernels = 7;
     static const void* const kern_of[kKernels] = {
         (const void*)k_cg_persistent<false>, (const void*)k_cg_persistent<true>,
         (const void*)k_cg_persistent<false, true, 0>, (const void*)k_cg_persistent<false, true, 1>,
-        (const void*)k_cg_persistent<false, false, 0>};
-    static int cached_dev[kKernels] = {-1, -1, -1, -1, -1}, cached[kKernels] = {0, 0, 0, 0, 0};
+        (const void*)k_cg_persistent<false, false, 0>, (const void*)k_cg_persistent<false, true, 0, true>,
+        (const void*)k_cg_persistent<false, true, 1, true>};
+    static int cached_dev[kKernels] = {-1, -1, -1, -1, -1, -1, -1}, cached[kKernels] = {0, 0, 0, 0, 0, 0, 0};
     int slot = 0;
     while (slot + 1 < kKernels && kern_of[slot] != kernel) ++slot;
     int dev = 0;
@@ -2932,6 +2944,17 @@ int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, Pro
     want = std::max<int64_t>(8, ceil_div(want, 8) * 8);
     if (P == 1 && plans[0]->grid_blocks > 0) want = ceil_div((int64_t)plans[0]->grid_blocks, 8) * 8;
     int nblocks = (int)std::min<int64_t>(std::min<int64_t>(coop_max, (int64_t)kMaxBlocks), want);
+    // a single problem with at most one node item per block runs the instance
+    // with the tagged barrier compiled in (each barrier kind compiled alone:
+    // a run-time choice between them cost 1-3 % per frame)
+    if (P == 1 && plans[0]->nchunks <= nblocks && nblocks <= 4 * kWave) {
+        const void* tk = interp_all == 0 ? (const void*)k_cg_persistent<false, true, 0, true>
+                                         : (const void*)k_cg_persistent<false, true, 1, true>;
+        int tag_max = 0;
+        rc = occupancy_blocks(tk, &tag_max);
+        if (rc != D2IC_OK) return rc;
+        if (tag_max >= nblocks) kern = tk;
+    }
     int max_steps = 8 + 2 * (2 + max_it * 12);
     ServeHost* no_sh = nullptr;
     void* args[] = {(void*)&d_desc, (void*)&P, (void*)&d_S, (void*)&d_ws, (void*)&max_steps, (void*)&no_sh};
@@ -2956,7 +2979,11 @@ int launch_solve(int P, d2ic_cg_plan* const* plans, const int32_t* max_iter, Pro
     D2IC_HIP(hipStreamIsCapturing(st, &cap));
     const bool plain_launch = plain_env || cap != hipStreamCaptureStatusNone;
     if (plain_launch) {
-        if (kern == (const void*)k_cg_persistent<false, true, 0>)
+        if (kern == (const void*)k_cg_persistent<false, true, 0, true>)
+            k_cg_persistent<false, true, 0, true><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        else if (kern == (const void*)k_cg_persistent<false, true, 1, true>)
+            k_cg_persistent<false, true, 1, true><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
+        else if (kern == (const void*)k_cg_persistent<false, true, 0>)
             k_cg_persistent<false, true, 0><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
         else if (kern == (const void*)k_cg_persistent<false, true, 1>)
             k_cg_persistent<false, true, 1><<<nblocks, kBlock, 0, st>>>(d_desc, P, d_S, d_ws, max_steps, nullptr);
