@@ -14,7 +14,7 @@ FP64 run for its x64 run).
 * 1024x1024 differentiation: the adjoint's image and reg_strength gradients
   and a noise-probe tangent against the FP64 oracle's finite differences.
 
-Bars (north star): integer ZNCC shifts bit-exact; nodal u within 1e-4 px of
+Bars: integer ZNCC shifts bit-exact; nodal u within 1e-4 px of
 the FP32 oracle; Green-Lagrange strain within 1e-6; the FP64 error is printed
 and bounded.  Linear mode at 2048^2 is the exception, with numbers: there the
 FP32 and FP64 oracle runs themselves differ by ~1e-3 px (the bilinear
